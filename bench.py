@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the finite-volume time step on one MI355X.
+
+Workload: 3D radial Sod, 256^3 uniform cells on [-1,1]^3, gamma 1.4, inside
+r<0.4 rho=1 p=1, outside rho=0.125 p=0.1, at rest, transmissive (free-flow)
+borders, first-order Rusanov + SSP-RK3 at CFL 0.5 towards t=0.25, FP64.
+
+A step is one SSP-RK3 time step (three fused flux+update stage kernels and
+the device-side dt/t update) over the whole grid, with the state resident
+in HBM.  value = cells x 3 stages x steps / wall time of the timed steps
+(cell-updates per second per RK stage).
+
+Prints ONE JSON line on stdout.  The roofline entry is for the stage kernel:
+achieved = algorithmic bytes per launch (80/120/120 B per cell for stages
+1/2/3, mean 106.67 B) / average launch duration from HIP events recorded on
+the solver's stream around every stage launch of the timed steps.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STAGE = (80.0 + 120.0 + 120.0) / 3.0
+
+
+def log(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n, steps):
+    """The oracle (plain C, one thread) on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    pb = O.problem(O.RADSOD, 3, sod_r0=0.4)
+    s = O.Solver(pb, 1, [n] * 3, [-1.0, -1.0, -1.0], 2.0 / n, side_bc=[O.BC_FREE_FLOW] * 6)
+    U = s.initial_state()
+    W = U.copy()
+    t = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt, _ = s.rk3_step(U, W, 0.5, t, 0.25)
+        t += dt
+    el = time.perf_counter() - t0
+    return {
+        "value": n ** 3 * 3 * steps / el,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "%d SSP-RK3 step(s) (%d stage updates of %d^3 cells) of the same radial-Sod workload "
+                  "from its initial condition, oracle/mmf_oracle.c -O2, single thread on %s; %.1f s"
+                  % (steps, 3 * steps, n, cpu_model(), el),
+    }
+
+
+def load_traffic(workload):
+    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--order", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design, not a measured path)" % args.gpus)
+        sys.exit(2)
+
+    import minimmerflow_amd as mmf
+
+    n = args.n
+    N = n ** 3
+    pb = mmf.problem("radsod", 3, sod_r0=0.4, domain_origin=[-1.0, -1.0, -1.0], domain_length=2.0)
+    s = mmf.Solver.for_problem(pb, n, order=args.order, side_bc=[mmf.BC_FREE_FLOW] * 6)
+    t_setup = time.perf_counter()
+    s.set_initial_state(pb)
+    s.set_time(0.0, 0.25, 0.5)
+    log("setup %.1fs: %d^3 cells, %d interfaces, order %d" % (time.perf_counter() - t_setup, n, s.ninterfaces, args.order))
+
+    s.advance(args.warmup)
+    s.synchronize()
+    s.enable_timing(True)
+    t0 = time.perf_counter()
+    s.advance(args.steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    stage_ms, launches = s.read_timing()
+    s.enable_timing(False)
+    t, dt, step, eig = s.time()
+    if step != args.warmup + args.steps:
+        raise SystemExit("bench: only %d of %d steps were active (t=%.4f reached t_end)" % (step, args.warmup + args.steps, t))
+    np_, nr_ = s.positivity_counters()
+    if np_ or nr_:
+        raise SystemExit("bench: negative pressure/density encountered (%d, %d)" % (np_, nr_))
+
+    value = N * 3 * args.steps / el
+    avg_ms = stage_ms / max(launches, 1)
+    achieved = BYTES_PER_CELL_STAGE * N / (avg_ms * 1e-3) / 1e9
+    workload = "radial_sod_3d_%d_order%d" % (n, args.order)
+    log("timed %d steps in %.3f s: %.3f ms/step, stage kernel avg %.4f ms (%d launches), %.0f GB/s, t=%.5f dt=%.3e"
+        % (args.steps, el, el / args.steps * 1e3, avg_ms, launches, achieved, t, dt))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle, %d step(s) at %d^3 ..." % (args.cpu_steps, n))
+        cpu = cpu_baseline(n, args.cpu_steps)
+        log("cpu baseline: %.3e cell-updates/s" % cpu["value"])
+
+    out = {
+        "metric": "cell-updates/s per RK stage",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (radial Sod initial condition sampled at cell centroids)",
+        "config": {
+            "workload": workload,
+            "cells": N,
+            "interfaces": s.ninterfaces,
+            "domain": "[-1,1]^3",
+            "initial": "r<0.4: rho=1 p=1; else rho=0.125 p=0.1; u=0",
+            "boundaries": "transmissive (free flow)",
+            "reconstruction": "order %d" % args.order,
+            "flux": "Rusanov",
+            "time": "SSP-RK3, CFL 0.5, t_end 0.25",
+            "t_after_timed_steps": t,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(workload),
+            "kernel": "k_stage (fused flux + RK stage), mean of stages 1-3",
+            "algorithmic_bytes_per_launch": BYTES_PER_CELL_STAGE * N,
+            "avg_launch_ms": avg_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+/*
+ * mmf.h -- C-ABI of the MI355X finite-volume time step (minimmerflow_amd).
+ *
+ * Plain C types only: pointers, sizes, status codes.  No C++ exception and no
+ * exit() crosses this boundary.  Every function returns MMF_OK (0) or an
+ * MMF_ERR_* code; mmf_last_error(ctx) holds a message for the last failure on
+ * that context.
+ *
+ * State layout at the boundary: host arrays of conservative variables in the
+ * reference's AoS order [cell][5] = (rho, rho*u, rho*v, rho*w, rho*E)
+ * (constants.hpp:43-47).  Cells are numbered i + nx*(j + ny*k) unless a raw
+ * order is installed with mmf_set_raw_order().  On the device the state is
+ * SoA (one plane per field) and stays resident between calls.
+ *
+ * One HIP stream per context.  Calls on one context must be serialised by
+ * the caller; distinct contexts are independent.
+ */
+#ifndef MMF_H
+#define MMF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMF_NF 5
+
+/* status codes; MMF_ERR_UNSUPPORTED_ORDER replaces the reference's exit(2)
+ * (reconstruction.cpp:75-76) */
+enum {
+    MMF_OK = 0,
+    MMF_ERR_INVALID = 1,
+    MMF_ERR_UNSUPPORTED_ORDER = 2,
+    MMF_ERR_HIP = 3,
+    MMF_ERR_NO_DEVICE = 4,
+    MMF_ERR_OUT_OF_MEMORY = 5,
+    MMF_ERR_STATE = 6
+};
+
+/* boundary condition codes: constants.hpp:58-62; PERIODIC is an extension */
+enum {
+    MMF_BC_NONE = -1,
+    MMF_BC_FREE_FLOW = 0,
+    MMF_BC_REFLECTING = 1,
+    MMF_BC_WALL = 2,
+    MMF_BC_DIRICHLET = 3,
+    MMF_BC_PERIODIC = 4
+};
+
+/* cell flag bits (mmf_set_cell_flags) */
+enum { MMF_CELL_FLUID = 1, MMF_CELL_SOLVED = 2 };
+
+/*
+ * Uniform Cartesian grid + boundary table: the closed-form replacement of
+ * MeshGeometricalInfo (mesh_info.hpp:31-105) and of the per-face BC storage
+ * built in computation() (main.cpp:251-277) with problem::getBorderBCType /
+ * getBorderBCInfo (problem.cpp:404-477).
+ *   cell volume = h^dim, face area = h^(dim-1), cell size = h,
+ *   centroid = origin + (i+1/2) h, face normals = +-e_axis (owner-outward).
+ * side order: x-, x+, y-, y+, z-, z+.  A periodic axis needs PERIODIC on
+ * both of its sides.  side_info holds primitive (p,u,v,w,T) Dirichlet data.
+ */
+typedef struct mmf_grid_desc {
+    int32_t dim;            /* 2 or 3 */
+    int32_t order;          /* 1 (reference) or 2 (limited linear) */
+    int64_t n[3];           /* cells per axis; n[2] ignored in 2D */
+    double origin[3];
+    double h;               /* cell edge length */
+    int32_t side_bc[6];
+    double side_info[6][MMF_NF];
+} mmf_grid_desc;
+
+typedef struct mmf_ctx mmf_ctx;
+
+/* ---- lifetime (MeshGeometricalInfo ctor + storage allocation,
+ *      main.cpp:197-219) -------------------------------------------------- */
+int mmf_create(const mmf_grid_desc *desc, mmf_ctx **out);
+int mmf_destroy(mmf_ctx *ctx);
+const char *mmf_last_error(const mmf_ctx *ctx);
+const char *mmf_status_string(int status);
+/* cells; interfaces as the reference counts them (border faces included) */
+int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *ninterfaces);
+
+/* ---- body / flags (body::isPointFluid + main.cpp:222-237, 251-277) ------- */
+/* fluid: [ncells] 0/1 or NULL (all fluid); solved: NULL -> solved = fluid */
+int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved);
+/* Marks cells on the device from their centroids: boxes [nb][6]
+ * (xmin,ymin,zmin,xmax,ymax,zmax; inclusive, body.cpp:80-95), spheres
+ * [ns][4] (cx,cy,cz,r; solid where |x-c| - r < 0).  solved = fluid. */
+int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nboxes,
+                   const double *spheres, int32_t nspheres);
+/* flags: [ncells] MMF_CELL_* bits */
+int mmf_get_cell_flags(mmf_ctx *ctx, uint8_t *flags);
+
+/* ---- state transfer (AoS host <-> SoA device) --------------------------- */
+int mmf_set_state(mmf_ctx *ctx, const double *U_aos);
+int mmf_get_state(mmf_ctx *ctx, double *U_aos);
+/* Optional: cell_of_raw[r] = structured index of the reference's raw cell r
+ * (bitpit PiercedStorage raw order).  After this, mmf_set_state /
+ * mmf_get_state / mmf_compute_rhs read and write host arrays in raw order. */
+int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw);
+
+/* ---- euler::computeRHS (euler.cpp:127-249) ------------------------------- */
+/* RHS of the current state (order-2 slopes included: the
+ * reconstruction::computePolynomials hook, reconstruction.cpp:47-55).
+ * rhs_aos: host [ncells][5] or NULL; max_eig: host scalar or NULL. */
+int mmf_compute_rhs(mmf_ctx *ctx, double *rhs_aos, double *max_eig);
+
+/* ---- RK3 time loop (main.cpp:374-506) ------------------------------------ */
+/* dt = 0.9 * cfl * h / maxEig(stage 1), clipped to tmax (main.cpp:398-402). */
+int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl);
+/* Enqueue nsteps SSP-RK3 steps; dt and t stay on the device and a step
+ * whose start time is >= tmax does nothing.  Does not synchronise. */
+int mmf_advance(mmf_ctx *ctx, int64_t nsteps);
+/* Step until t >= tmax (or max_steps when >= 0); synchronises. */
+int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_final);
+/* Synchronises and reads the time control block.  stage_eig: the three
+ * stage maxima of the last step (main.cpp:389,433,469) or NULL. */
+int mmf_get_time(mmf_ctx *ctx, double *t, double *dt, int64_t *step, double *stage_eig);
+int mmf_synchronize(mmf_ctx *ctx);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+/* main.cpp:550-561: sum |rho - rho_exact| V over all cells; l2 =
+ * sqrt(sum (rho - rho_exact)^2 V) (extension).  rho_exact: host [ncells]. */
+int mmf_error_norms(mmf_ctx *ctx, const double *rho_exact, double *l1, double *l2);
+/* euler.cpp:94-101 logs negative pressure / density per flux evaluation;
+ * here they are counted (per interface side, as the reference evaluates). */
+int mmf_get_positivity_counters(mmf_ctx *ctx, int64_t *neg_p, int64_t *neg_rho);
+int mmf_reset_positivity_counters(mmf_ctx *ctx);
+/* Kernel timing with HIP events on the context's stream.  While enabled,
+ * every stage-kernel launch is bracketed by an event pair. */
+int mmf_enable_timing(mmf_ctx *ctx, int32_t on);
+/* total device milliseconds and launch count of the stage kernel since
+ * timing was enabled (synchronises) */
+int mmf_read_timing(mmf_ctx *ctx, double *stage_ms, int64_t *stage_launches);
+
+/* ---- point functions, batched on the device (host arrays in/out) --------- */
+/* utils::conservative2primitive / primitive2conservative (utils.cpp:48-83) */
+int mmf_batch_cons2prim(const double *cons, int64_t count, double *prim);
+int mmf_batch_prim2cons(const double *prim, int64_t count, double *cons);
+/* euler::evalFluxes (euler.cpp:83-113): normals [count][3] */
+int mmf_batch_eval_fluxes(const double *cons, const double *prim, const double *normals,
+                          int64_t count, double *flux);
+/* euler::evalSplitting (euler.cpp:42-73) */
+int mmf_batch_eval_splitting(const double *consL, const double *consR, const double *normals,
+                             int64_t count, double *flux, double *lambda);
+/* euler::evalInterfaceBCValues (euler.cpp:261-376); info [count][5] or NULL */
+int mmf_batch_eval_bc_values(int32_t bc, const double *normals, const double *info,
+                             const double *cons, int64_t count, double *cons_bc);
+
+/* ---- problem catalogue (problem.cpp), host-side setup -------------------- */
+/* problem.hpp:33-42; FREESTREAM (uniform state, Dirichlet inflow at x-) is an
+ * extension for the immersed-body case */
+enum {
+    MMF_PROBLEM_VORTEX_XY = 0,
+    MMF_PROBLEM_VORTEX_ZX = 1,
+    MMF_PROBLEM_VORTEX_YZ = 2,
+    MMF_PROBLEM_RADSOD = 3,
+    MMF_PROBLEM_SOD_X = 4,
+    MMF_PROBLEM_SOD_Y = 5,
+    MMF_PROBLEM_SOD_Z = 6,
+    MMF_PROBLEM_FFSTEP = 7,
+    MMF_PROBLEM_FREESTREAM = 8
+};
+
+/* Defaults are the reference's hard-wired values (problem.cpp:70-149,
+ * 257-262, 336-342, 383-387); the other fields parametrise the variants. */
+typedef struct mmf_problem {
+    int32_t type;
+    int32_t dim;
+    double vortex_beta;          /* problem.cpp:257 */
+    double vortex_center[3];
+    int32_t vortex_periodic;     /* exact solution wrapped on a periodic box */
+    double domain_origin[3];
+    double domain_length;
+    double sod_r0;               /* problem.cpp:336 */
+    double sod_p_in, sod_t_in;
+    double sod_p_out, sod_t_out;
+    double fs_prim[MMF_NF];      /* free stream / Dirichlet primitives */
+} mmf_problem;
+
+int mmf_problem_defaults(mmf_problem *pb, int32_t type, int32_t dim);
+/* problem.cpp:175-211 */
+int mmf_problem_default_end_time(const mmf_problem *pb, double *tend);
+/* problem::evalExactConservatives (problem.cpp:247-392) at count points */
+int mmf_problem_exact(const mmf_problem *pb, const double *points, int64_t count, double t,
+                      double *cons);
+/* problem::getBorderBCType / getBorderBCInfo (problem.cpp:404-477) */
+int mmf_problem_border_bc(const mmf_problem *pb, const double *face_centroid, int32_t *bc,
+                          double *info);
+/* side_bc / side_info of desc from the problem's border rule */
+int mmf_problem_side_bcs(const mmf_problem *pb, mmf_grid_desc *desc);
+/* main.cpp:335-344: upload the exact solution at t=0 sampled at centroids */
+int mmf_set_initial_state(mmf_ctx *ctx, const mmf_problem *pb);
+/* main.cpp:550-561 against the problem's exact density at time t */
+int mmf_error_vs_exact(mmf_ctx *ctx, const mmf_problem *pb, double t, double *l1, double *l2);
+/* cell centroids [ncells][3] (origin + (i+1/2) h; z = origin z in 2D) in the
+ * current host order */
+int mmf_get_centroids(const mmf_ctx *ctx, double *xyz);
+
+/* ---- bandwidth calibration ------------------------------------------------ */
+/* Copies nbytes device-to-device with a plain vector-load/store kernel reps
+ * times; returns the average kernel milliseconds (HIP events). */
+int mmf_calibration_copy(int64_t nbytes, int32_t reps, double *ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MMF_H */

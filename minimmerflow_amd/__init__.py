@@ -1,0 +1,385 @@
+"""minimmerflow_amd -- MI355X-native finite-volume time step of minimmerflow.
+
+Python front-end over the C-ABI of lib/libmmf.so (include/mmf.h).  Every
+compute call runs the hand-written HIP kernels; there is no CPU fallback: if
+the library is missing or no GPU is present the calls raise.
+
+The classes mirror the reference's entry points:
+  Solver.compute_rhs   -> euler::computeRHS            (euler.cpp:127-249)
+  Solver.advance/run   -> the RK3 loop of computation() (main.cpp:374-506)
+  Solver.set_bodies    -> body::isPointFluid + flags   (body.cpp:80-95, main.cpp:222-277)
+  Problem              -> problem::*                   (problem.cpp)
+  batch_* functions    -> utils::* / euler::eval*      (utils.cpp, euler.cpp)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmmf.so")
+
+NF = 5
+OK, ERR_INVALID, ERR_UNSUPPORTED_ORDER, ERR_HIP, ERR_NO_DEVICE, ERR_OOM, ERR_STATE = range(7)
+BC_NONE, BC_FREE_FLOW, BC_REFLECTING, BC_WALL, BC_DIRICHLET, BC_PERIODIC = -1, 0, 1, 2, 3, 4
+CELL_FLUID, CELL_SOLVED = 1, 2
+(VORTEX_XY, VORTEX_ZX, VORTEX_YZ, RADSOD, SOD_X, SOD_Y, SOD_Z, FFSTEP, FREESTREAM) = range(9)
+PROBLEM_KEYS = {"vortex_xy": VORTEX_XY, "vortex_zx": VORTEX_ZX, "vortex_yz": VORTEX_YZ,
+                "radsod": RADSOD, "sod3d_x": SOD_X, "sod3d_y": SOD_Y, "sod3d_z": SOD_Z,
+                "ffstep": FFSTEP, "freestream": FREESTREAM}
+
+
+class MMFError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__("mmf status %d (%s)%s" % (status, _status_string(status), (": " + msg) if msg else ""))
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("order", C.c_int32), ("n", C.c_int64 * 3),
+                ("origin", C.c_double * 3), ("h", C.c_double), ("side_bc", C.c_int32 * 6),
+                ("side_info", (C.c_double * 5) * 6)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("type", C.c_int32), ("dim", C.c_int32), ("vortex_beta", C.c_double),
+                ("vortex_center", C.c_double * 3), ("vortex_periodic", C.c_int32),
+                ("domain_origin", C.c_double * 3), ("domain_length", C.c_double),
+                ("sod_r0", C.c_double), ("sod_p_in", C.c_double), ("sod_t_in", C.c_double),
+                ("sod_p_out", C.c_double), ("sod_t_out", C.c_double), ("fs_prim", C.c_double * 5)]
+
+
+_lib = None
+
+
+def _status_string(s):
+    try:
+        return lib().mmf_status_string(int(s)).decode()
+    except Exception:  # library not loadable: plain code
+        return str(s)
+
+
+def lib():
+    """Load lib/libmmf.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("minimmerflow_amd: %s is missing; build it with "
+                          "`python -m minimmerflow_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, dp, i64p, u8p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint8)
+    i32p = C.POINTER(C.c_int32)
+    sig = {
+        "mmf_create": ([C.POINTER(GridDesc), C.POINTER(C.c_void_p)], C.c_int),
+        "mmf_destroy": ([vp], C.c_int),
+        "mmf_last_error": ([vp], C.c_char_p),
+        "mmf_status_string": ([C.c_int], C.c_char_p),
+        "mmf_get_sizes": ([vp, i64p, i64p], C.c_int),
+        "mmf_set_cell_flags": ([vp, u8p, u8p], C.c_int),
+        "mmf_set_bodies": ([vp, dp, C.c_int32, dp, C.c_int32], C.c_int),
+        "mmf_get_cell_flags": ([vp, u8p], C.c_int),
+        "mmf_set_state": ([vp, dp], C.c_int),
+        "mmf_get_state": ([vp, dp], C.c_int),
+        "mmf_set_raw_order": ([vp, i64p, C.c_int64], C.c_int),
+        "mmf_compute_rhs": ([vp, dp, dp], C.c_int),
+        "mmf_set_time": ([vp, C.c_double, C.c_double, C.c_double], C.c_int),
+        "mmf_advance": ([vp, C.c_int64], C.c_int),
+        "mmf_run": ([vp, C.c_int64, i64p, dp], C.c_int),
+        "mmf_get_time": ([vp, dp, dp, i64p, dp], C.c_int),
+        "mmf_synchronize": ([vp], C.c_int),
+        "mmf_error_norms": ([vp, dp, dp, dp], C.c_int),
+        "mmf_get_positivity_counters": ([vp, i64p, i64p], C.c_int),
+        "mmf_reset_positivity_counters": ([vp], C.c_int),
+        "mmf_enable_timing": ([vp, C.c_int32], C.c_int),
+        "mmf_read_timing": ([vp, dp, i64p], C.c_int),
+        "mmf_batch_cons2prim": ([dp, C.c_int64, dp], C.c_int),
+        "mmf_batch_prim2cons": ([dp, C.c_int64, dp], C.c_int),
+        "mmf_batch_eval_fluxes": ([dp, dp, dp, C.c_int64, dp], C.c_int),
+        "mmf_batch_eval_splitting": ([dp, dp, dp, C.c_int64, dp, dp], C.c_int),
+        "mmf_batch_eval_bc_values": ([C.c_int32, dp, dp, dp, C.c_int64, dp], C.c_int),
+        "mmf_problem_defaults": ([C.POINTER(Problem), C.c_int32, C.c_int32], C.c_int),
+        "mmf_problem_default_end_time": ([C.POINTER(Problem), dp], C.c_int),
+        "mmf_problem_exact": ([C.POINTER(Problem), dp, C.c_int64, C.c_double, dp], C.c_int),
+        "mmf_problem_border_bc": ([C.POINTER(Problem), dp, i32p, dp], C.c_int),
+        "mmf_problem_side_bcs": ([C.POINTER(Problem), C.POINTER(GridDesc)], C.c_int),
+        "mmf_set_initial_state": ([vp, C.POINTER(Problem)], C.c_int),
+        "mmf_error_vs_exact": ([vp, C.POINTER(Problem), C.c_double, dp, dp], C.c_int),
+        "mmf_get_centroids": ([vp, dp], C.c_int),
+        "mmf_calibration_copy": ([C.c_int64, C.c_int32, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the header declares (for the ABI test)."""
+    import re
+    hdr = open(os.path.join(os.path.dirname(HERE), "include", "mmf.h")).read()
+    return sorted(set(re.findall(r"\b(mmf_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _check(rc, ctx=None):
+    if rc != OK:
+        msg = ""
+        if ctx:
+            m = lib().mmf_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise MMFError(rc, msg)
+
+
+# ---- problem catalogue -----------------------------------------------------------
+
+def problem(ptype, dim=2, **overrides):
+    """Problem with the reference defaults (problem.cpp) and optional overrides."""
+    if isinstance(ptype, str):
+        ptype = PROBLEM_KEYS[ptype]
+    pb = Problem()
+    _check(lib().mmf_problem_defaults(C.byref(pb), int(ptype), int(dim)))
+    for k, v in overrides.items():
+        if isinstance(v, (list, tuple, np.ndarray)):
+            arr = getattr(pb, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(pb, k, v)
+    return pb
+
+
+def default_end_time(pb):
+    t = C.c_double()
+    _check(lib().mmf_problem_default_end_time(C.byref(pb), C.byref(t)))
+    return t.value
+
+
+def exact_conservatives(pb, points, t):
+    pts = _f64(points).reshape(-1, 3)
+    out = np.empty((pts.shape[0], NF))
+    _check(lib().mmf_problem_exact(C.byref(pb), _dp(pts), pts.shape[0], float(t), _dp(out)))
+    return out
+
+
+# ---- batched point functions ------------------------------------------------------
+
+def batch_cons2prim(cons):
+    c = _f64(cons).reshape(-1, NF)
+    out = np.empty_like(c)
+    _check(lib().mmf_batch_cons2prim(_dp(c), c.shape[0], _dp(out)))
+    return out
+
+
+def batch_prim2cons(prim):
+    p = _f64(prim).reshape(-1, NF)
+    out = np.empty_like(p)
+    _check(lib().mmf_batch_prim2cons(_dp(p), p.shape[0], _dp(out)))
+    return out
+
+
+def batch_eval_fluxes(cons, prim, normals):
+    c, p, n = _f64(cons).reshape(-1, NF), _f64(prim).reshape(-1, NF), _f64(normals).reshape(-1, 3)
+    out = np.empty_like(c)
+    _check(lib().mmf_batch_eval_fluxes(_dp(c), _dp(p), _dp(n), c.shape[0], _dp(out)))
+    return out
+
+
+def batch_eval_splitting(L, R, normals):
+    L, R, n = _f64(L).reshape(-1, NF), _f64(R).reshape(-1, NF), _f64(normals).reshape(-1, 3)
+    F = np.empty_like(L)
+    lam = np.empty(L.shape[0])
+    _check(lib().mmf_batch_eval_splitting(_dp(L), _dp(R), _dp(n), L.shape[0], _dp(F), _dp(lam)))
+    return F, lam
+
+
+def batch_eval_bc_values(bc, normals, cons, info=None):
+    n, c = _f64(normals).reshape(-1, 3), _f64(cons).reshape(-1, NF)
+    i = None if info is None else _f64(info).reshape(-1, NF)
+    out = np.empty_like(c)
+    _check(lib().mmf_batch_eval_bc_values(int(bc), _dp(n), None if i is None else _dp(i), _dp(c),
+                                          c.shape[0], _dp(out)))
+    return out
+
+
+def calibration_copy(nbytes, reps=20):
+    ms = C.c_double()
+    _check(lib().mmf_calibration_copy(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
+
+
+# ---- solver ------------------------------------------------------------------------
+
+class Solver:
+    """Device-resident solver of one uniform Cartesian grid."""
+
+    def __init__(self, dim, n, origin, h, side_bc, side_info=None, order=1):
+        d = GridDesc()
+        d.dim = int(dim)
+        d.order = int(order)
+        n = list(n) + [1] * (3 - len(n))
+        for a in range(3):
+            d.n[a] = int(n[a])
+            d.origin[a] = float(origin[a]) if a < len(origin) else 0.0
+        d.h = float(h)
+        for s in range(6):
+            d.side_bc[s] = int(side_bc[s]) if s < len(side_bc) else BC_FREE_FLOW
+            if side_info is not None:
+                for k in range(NF):
+                    d.side_info[s][k] = float(side_info[s][k])
+        self.desc = d
+        self.dim = int(dim)
+        self.n = tuple(int(x) for x in n)
+        self.h = float(h)
+        self.order = int(order)
+        ctx = C.c_void_p()
+        _check(lib().mmf_create(C.byref(d), C.byref(ctx)))
+        self.ctx = ctx
+        nc, nf = C.c_int64(), C.c_int64()
+        _check(lib().mmf_get_sizes(self.ctx, C.byref(nc), C.byref(nf)))
+        self.ncells, self.ninterfaces = nc.value, nf.value
+
+    @classmethod
+    def for_problem(cls, pb, ncells, order=1, side_bc=None, origin=None, length=None):
+        """Grid as computation() builds it: length/nCells cells (main.cpp:147-149),
+        border BCs from the problem (main.cpp:251-260) unless given."""
+        dim = pb.dim
+        origin = list(pb.domain_origin) if origin is None else list(origin)
+        length = pb.domain_length if length is None else float(length)
+        n = [int(ncells)] * dim
+        h = length / int(ncells)
+        d = GridDesc()
+        d.dim, d.order, d.h = dim, order, h
+        for a in range(3):
+            d.n[a] = n[a] if a < dim else 1
+            d.origin[a] = origin[a]
+        _check(lib().mmf_problem_side_bcs(C.byref(pb), C.byref(d)))
+        bcs = [d.side_bc[s] for s in range(6)] if side_bc is None else list(side_bc)
+        info = [[d.side_info[s][k] for k in range(NF)] for s in range(6)]
+        return cls(dim, n, origin, h, bcs, info, order)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().mmf_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(rc, self.ctx)
+
+    # state
+    def set_state(self, U):
+        U = _f64(U).reshape(self.ncells, NF)
+        self._c(lib().mmf_set_state(self.ctx, _dp(U)))
+
+    def get_state(self):
+        U = np.empty((self.ncells, NF))
+        self._c(lib().mmf_get_state(self.ctx, _dp(U)))
+        return U
+
+    def set_initial_state(self, pb):
+        self._c(lib().mmf_set_initial_state(self.ctx, C.byref(pb)))
+
+    def centroids(self):
+        out = np.empty((self.ncells, 3))
+        self._c(lib().mmf_get_centroids(self.ctx, _dp(out)))
+        return out
+
+    def set_raw_order(self, cell_of_raw):
+        if cell_of_raw is None:
+            self._c(lib().mmf_set_raw_order(self.ctx, None, 0))
+            return
+        a = np.ascontiguousarray(cell_of_raw, dtype=np.int64)
+        self._raw = a
+        self._c(lib().mmf_set_raw_order(self.ctx, a.ctypes.data_as(C.POINTER(C.c_int64)), a.size))
+
+    # bodies / flags
+    def set_cell_flags(self, fluid, solved=None):
+        f = np.ascontiguousarray(fluid, dtype=np.uint8)
+        s = None if solved is None else np.ascontiguousarray(solved, dtype=np.uint8)
+        self._c(lib().mmf_set_cell_flags(self.ctx, f.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                         None if s is None else s.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def set_bodies(self, boxes=(), spheres=()):
+        b = _f64(np.asarray(boxes, dtype=np.float64).reshape(-1, 6))
+        s = _f64(np.asarray(spheres, dtype=np.float64).reshape(-1, 4))
+        self._c(lib().mmf_set_bodies(self.ctx, _dp(b) if len(b) else None, len(b),
+                                     _dp(s) if len(s) else None, len(s)))
+
+    def cell_flags(self):
+        out = np.empty(self.ncells, dtype=np.uint8)
+        self._c(lib().mmf_get_cell_flags(self.ctx, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    # euler::computeRHS
+    def compute_rhs(self, want_rhs=True):
+        R = np.empty((self.ncells, NF)) if want_rhs else None
+        me = C.c_double()
+        self._c(lib().mmf_compute_rhs(self.ctx, _dp(R) if want_rhs else None, C.byref(me)))
+        return R, me.value
+
+    # time loop
+    def set_time(self, t, tmax, cfl):
+        self._c(lib().mmf_set_time(self.ctx, float(t), float(tmax), float(cfl)))
+
+    def advance(self, nsteps):
+        self._c(lib().mmf_advance(self.ctx, int(nsteps)))
+
+    def run(self, max_steps=-1):
+        steps, t = C.c_int64(), C.c_double()
+        self._c(lib().mmf_run(self.ctx, int(max_steps), C.byref(steps), C.byref(t)))
+        return steps.value, t.value
+
+    def time(self):
+        t, dt, step = C.c_double(), C.c_double(), C.c_int64()
+        eig = np.empty(3)
+        self._c(lib().mmf_get_time(self.ctx, C.byref(t), C.byref(dt), C.byref(step), _dp(eig)))
+        return t.value, dt.value, step.value, eig
+
+    def synchronize(self):
+        self._c(lib().mmf_synchronize(self.ctx))
+
+    # diagnostics
+    def error_norms(self, rho_exact):
+        r = _f64(rho_exact).reshape(self.ncells)
+        l1, l2 = C.c_double(), C.c_double()
+        self._c(lib().mmf_error_norms(self.ctx, _dp(r), C.byref(l1), C.byref(l2)))
+        return l1.value, l2.value
+
+    def error_vs_exact(self, pb, t):
+        l1, l2 = C.c_double(), C.c_double()
+        self._c(lib().mmf_error_vs_exact(self.ctx, C.byref(pb), float(t), C.byref(l1), C.byref(l2)))
+        return l1.value, l2.value
+
+    def positivity_counters(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._c(lib().mmf_get_positivity_counters(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def reset_positivity_counters(self):
+        self._c(lib().mmf_reset_positivity_counters(self.ctx))
+
+    def enable_timing(self, on=True):
+        self._c(lib().mmf_enable_timing(self.ctx, 1 if on else 0))
+
+    def read_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        self._c(lib().mmf_read_timing(self.ctx, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

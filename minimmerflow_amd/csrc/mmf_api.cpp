@@ -1,0 +1,645 @@
+// mmf_api.cpp -- C-ABI of minimmerflow_amd (include/mmf.h).
+//
+// Owns the device-resident state of one structured grid: SoA conservative
+// state U, residual R, stage state W, optional cell flags and the time-control
+// block.  No exception leaves this file: every entry point returns a status.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mmf.h"
+#include "mmf_internal.hpp"
+
+using mmf::Ctrl;
+using mmf::GridDev;
+
+struct mmf_ctx {
+  mmf_grid_desc desc;
+  GridDev g;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double *dU = nullptr, *dR = nullptr, *dW = nullptr;
+  uint8_t *dflags = nullptr;
+  Ctrl *dctrl = nullptr;
+  Ctrl *hctrl = nullptr;  // pinned mirror for read-back
+  double *dpartials = nullptr;
+  int64_t nerr_blocks = 0;
+  double *derr = nullptr;
+  std::vector<int64_t> raw;  // cell_of_raw, empty when identity
+  std::string err;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  double timed_ms = 0.0;
+  int64_t timed_launches = 0;
+};
+
+namespace {
+
+int fail(mmf_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(mmf_ctx *ctx, hipError_t e, const char *what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, MMF_ERR_HIP, m);
+}
+
+#define HIPCHK(ctx, expr)                               \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+bool valid_bc(int b) { return b >= MMF_BC_FREE_FLOW && b <= MMF_BC_PERIODIC; }
+
+// AoS host (optionally in raw order) -> SoA host
+void aos_to_soa(const mmf_ctx *ctx, const double *aos, std::vector<double> &soa) {
+  const int64_t N = ctx->g.N, fs = ctx->g.fs;
+  soa.assign((size_t)(5 * fs), 0.0);
+  if (ctx->raw.empty()) {
+    for (int64_t c = 0; c < N; ++c)
+      for (int k = 0; k < 5; ++k) soa[(size_t)(k * fs + c)] = aos[5 * c + k];
+  } else {
+    for (int64_t r = 0; r < N; ++r) {
+      int64_t c = ctx->raw[(size_t)r];
+      for (int k = 0; k < 5; ++k) soa[(size_t)(k * fs + c)] = aos[5 * r + k];
+    }
+  }
+}
+
+void soa_to_aos(const mmf_ctx *ctx, const std::vector<double> &soa, double *aos) {
+  const int64_t N = ctx->g.N, fs = ctx->g.fs;
+  if (ctx->raw.empty()) {
+    for (int64_t c = 0; c < N; ++c)
+      for (int k = 0; k < 5; ++k) aos[5 * c + k] = soa[(size_t)(k * fs + c)];
+  } else {
+    for (int64_t r = 0; r < N; ++r) {
+      int64_t c = ctx->raw[(size_t)r];
+      for (int k = 0; k < 5; ++k) aos[5 * r + k] = soa[(size_t)(k * fs + c)];
+    }
+  }
+}
+
+int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ctx->timing) {
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+  }
+  hipError_t e = mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
+                                   ctx->dctrl, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "stage kernel launch");
+  if (ctx->timing) {
+    HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+    ctx->events.emplace_back(e0, e1);
+  }
+  return MMF_OK;
+}
+
+int collect_timing(mmf_ctx *ctx) {
+  for (auto &p : ctx->events) {
+    HIPCHK(ctx, hipEventSynchronize(p.second));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, p.first, p.second));
+    ctx->timed_ms += ms;
+    ctx->timed_launches += 1;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  ctx->events.clear();
+  return MMF_OK;
+}
+
+int read_ctrl(mmf_ctx *ctx) {
+  HIPCHK(ctx, hipMemcpyAsync(ctx->hctrl, ctx->dctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mmf_status_string(int s) {
+  switch (s) {
+    case MMF_OK: return "ok";
+    case MMF_ERR_INVALID: return "invalid argument";
+    case MMF_ERR_UNSUPPORTED_ORDER: return "unsupported reconstruction order";
+    case MMF_ERR_HIP: return "HIP runtime error";
+    case MMF_ERR_NO_DEVICE: return "no HIP device";
+    case MMF_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case MMF_ERR_STATE: return "invalid state";
+    default: return "unknown status";
+  }
+}
+
+const char *mmf_last_error(const mmf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
+  if (!d || !out) return MMF_ERR_INVALID;
+  *out = nullptr;
+  if (d->dim != 2 && d->dim != 3) return MMF_ERR_INVALID;
+  if (d->order != 1 && d->order != 2) return MMF_ERR_UNSUPPORTED_ORDER;
+  if (!(d->h > 0.0)) return MMF_ERR_INVALID;
+  for (int a = 0; a < d->dim; ++a) {
+    if (d->n[a] < 1) return MMF_ERR_INVALID;
+    int lo = d->side_bc[2 * a], hi = d->side_bc[2 * a + 1];
+    if (!valid_bc(lo) || !valid_bc(hi)) return MMF_ERR_INVALID;
+    if ((lo == MMF_BC_PERIODIC) != (hi == MMF_BC_PERIODIC)) return MMF_ERR_INVALID;
+    if (lo == MMF_BC_PERIODIC && d->n[a] < 2) return MMF_ERR_INVALID;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MMF_ERR_NO_DEVICE;
+
+  mmf_ctx *ctx = new (std::nothrow) mmf_ctx();
+  if (!ctx) return MMF_ERR_OUT_OF_MEMORY;
+  ctx->desc = *d;
+  GridDev &g = ctx->g;
+  std::memset(&g, 0, sizeof(g));
+  g.dim = d->dim;
+  g.order = d->order;
+  for (int a = 0; a < 3; ++a) g.n[a] = (a < d->dim) ? d->n[a] : 1;
+  g.N = g.n[0] * g.n[1] * g.n[2];
+  g.fs = (g.N + 63) / 64 * 64;
+  g.h = d->h;
+  g.V = (d->dim == 3) ? d->h * d->h * d->h : d->h * d->h;
+  g.A = (d->dim == 3) ? d->h * d->h : d->h;
+  for (int s = 0; s < 6; ++s) {
+    g.bc[s] = (s / 2 < d->dim) ? d->side_bc[s] : MMF_BC_FREE_FLOW;
+    for (int k = 0; k < 5; ++k) g.info[s][k] = d->side_info[s][k];
+  }
+  for (int a = 0; a < 3; ++a) g.periodic[a] = (a < d->dim) && d->side_bc[2 * a] == MMF_BC_PERIODIC;
+  g.flags = nullptr;
+
+  auto cleanup = [&](int code) {
+    mmf_destroy(ctx);
+    return code;
+  };
+  if (hipGetDevice(&ctx->device) != hipSuccess) return cleanup(MMF_ERR_NO_DEVICE);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(MMF_ERR_HIP);
+  const size_t state_bytes = sizeof(double) * 5 * (size_t)g.fs;
+  if (hipMalloc(&ctx->dU, state_bytes) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&ctx->dR, state_bytes) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&ctx->dW, state_bytes) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&ctx->dctrl, sizeof(Ctrl)) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipHostMalloc(&ctx->hctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  ctx->nerr_blocks = mmf::error_norm_blocks(g.N);
+  if (hipMalloc(&ctx->dpartials, sizeof(double) * 2 * (size_t)ctx->nerr_blocks) != hipSuccess)
+    return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&ctx->derr, sizeof(double) * 2) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipMemsetAsync(ctx->dU, 0, state_bytes, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->dR, 0, state_bytes, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->dW, 0, state_bytes, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(ctx->dctrl, 0, sizeof(Ctrl), ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return cleanup(MMF_ERR_HIP);
+  std::memset(ctx->hctrl, 0, sizeof(Ctrl));
+  *out = ctx;
+  return MMF_OK;
+}
+
+int mmf_destroy(mmf_ctx *ctx) {
+  if (!ctx) return MMF_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &p : ctx->events) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  if (ctx->dU) (void)hipFree(ctx->dU);
+  if (ctx->dR) (void)hipFree(ctx->dR);
+  if (ctx->dW) (void)hipFree(ctx->dW);
+  if (ctx->dflags) (void)hipFree(ctx->dflags);
+  if (ctx->dctrl) (void)hipFree(ctx->dctrl);
+  if (ctx->hctrl) (void)hipHostFree(ctx->hctrl);
+  if (ctx->dpartials) (void)hipFree(ctx->dpartials);
+  if (ctx->derr) (void)hipFree(ctx->derr);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MMF_OK;
+}
+
+int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *nifaces) {
+  if (!ctx) return MMF_ERR_INVALID;
+  const GridDev &g = ctx->g;
+  if (ncells) *ncells = g.N;
+  if (nifaces) {
+    int64_t nf = 0;
+    for (int a = 0; a < g.dim; ++a) nf += (g.periodic[a] ? g.n[a] : g.n[a] + 1) * (g.N / g.n[a]);
+    *nifaces = nf;
+  }
+  return MMF_OK;
+}
+
+int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved) {
+  if (!ctx) return MMF_ERR_INVALID;
+  const int64_t N = ctx->g.N;
+  std::vector<uint8_t> f((size_t)N);
+  bool all = true;
+  for (int64_t r = 0; r < N; ++r) {
+    uint8_t fl = fluid ? (fluid[r] ? 1 : 0) : 1;
+    uint8_t so = solved ? (solved[r] ? 1 : 0) : fl;
+    if (so && !fl) return fail(ctx, MMF_ERR_INVALID, "a solved cell must be fluid (main.cpp:230-236)");
+    uint8_t v = (uint8_t)((fl ? MMF_CELL_FLUID : 0) | (so ? MMF_CELL_SOLVED : 0));
+    int64_t c = ctx->raw.empty() ? r : ctx->raw[(size_t)r];
+    f[(size_t)c] = v;
+    all = all && (v == (MMF_CELL_FLUID | MMF_CELL_SOLVED));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (all) {
+    if (ctx->dflags) (void)hipFree(ctx->dflags);
+    ctx->dflags = nullptr;
+    ctx->g.flags = nullptr;
+    return MMF_OK;
+  }
+  if (!ctx->dflags) HIPCHK(ctx, hipMalloc(&ctx->dflags, (size_t)N));
+  HIPCHK(ctx, hipMemcpy(ctx->dflags, f.data(), (size_t)N, hipMemcpyHostToDevice));
+  ctx->g.flags = ctx->dflags;
+  return MMF_OK;
+}
+
+int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nb, const double *spheres, int32_t ns) {
+  if (!ctx || nb < 0 || ns < 0 || (nb && !boxes) || (ns && !spheres)) return fail(ctx, MMF_ERR_INVALID, "bodies");
+  const int64_t N = ctx->g.N;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  uint8_t *dfl = ctx->dflags;
+  if (!dfl) HIPCHK(ctx, hipMalloc(&dfl, (size_t)N));
+  double *dbox = nullptr, *dsph = nullptr;
+  if (nb) {
+    HIPCHK(ctx, hipMalloc(&dbox, sizeof(double) * 6 * nb));
+    HIPCHK(ctx, hipMemcpy(dbox, boxes, sizeof(double) * 6 * nb, hipMemcpyHostToDevice));
+  }
+  if (ns) {
+    HIPCHK(ctx, hipMalloc(&dsph, sizeof(double) * 4 * ns));
+    HIPCHK(ctx, hipMemcpy(dsph, spheres, sizeof(double) * 4 * ns, hipMemcpyHostToDevice));
+  }
+  GridDev g = ctx->g;
+  hipError_t e = mmf::launch_mark_bodies(g, ctx->desc.origin, dbox, nb, dsph, ns, dfl, ctx->stream);
+  std::vector<uint8_t> h((size_t)N);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dfl, (size_t)N, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (dbox) (void)hipFree(dbox);
+  if (dsph) (void)hipFree(dsph);
+  if (e != hipSuccess) {
+    if (dfl != ctx->dflags) (void)hipFree(dfl);
+    return hip_fail(ctx, e, "body marking");
+  }
+  bool all = std::all_of(h.begin(), h.end(), [](uint8_t v) { return v == (MMF_CELL_FLUID | MMF_CELL_SOLVED); });
+  if (all) {
+    (void)hipFree(dfl);
+    ctx->dflags = nullptr;
+    ctx->g.flags = nullptr;
+  } else {
+    ctx->dflags = dfl;
+    ctx->g.flags = dfl;
+  }
+  return MMF_OK;
+}
+
+int mmf_get_cell_flags(mmf_ctx *ctx, uint8_t *flags) {
+  if (!ctx || !flags) return MMF_ERR_INVALID;
+  const int64_t N = ctx->g.N;
+  std::vector<uint8_t> h((size_t)N, (uint8_t)(MMF_CELL_FLUID | MMF_CELL_SOLVED));
+  if (ctx->dflags) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(h.data(), ctx->dflags, (size_t)N, hipMemcpyDeviceToHost));
+  }
+  for (int64_t r = 0; r < N; ++r) flags[r] = h[(size_t)(ctx->raw.empty() ? r : ctx->raw[(size_t)r])];
+  return MMF_OK;
+}
+
+int mmf_get_centroids(const mmf_ctx *ctx, double *xyz) {
+  if (!ctx || !xyz) return MMF_ERR_INVALID;
+  const GridDev &g = ctx->g;
+  const double *o = ctx->desc.origin;
+  for (int64_t r = 0; r < g.N; ++r) {
+    int64_t c = ctx->raw.empty() ? r : ctx->raw[(size_t)r];
+    int64_t q[3] = {c % g.n[0], (c / g.n[0]) % g.n[1], c / (g.n[0] * g.n[1])};
+    for (int d = 0; d < 3; ++d) xyz[3 * r + d] = (d < g.dim) ? o[d] + (q[d] + 0.5) * g.h : o[d];
+  }
+  return MMF_OK;
+}
+
+int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw) {
+  if (!ctx) return MMF_ERR_INVALID;
+  if (!cell_of_raw) {
+    ctx->raw.clear();
+    return MMF_OK;
+  }
+  if (nraw != ctx->g.N) return fail(ctx, MMF_ERR_INVALID, "raw order size must equal the cell count");
+  std::vector<uint8_t> seen((size_t)nraw, 0);
+  for (int64_t r = 0; r < nraw; ++r) {
+    int64_t c = cell_of_raw[r];
+    if (c < 0 || c >= nraw || seen[(size_t)c]) return fail(ctx, MMF_ERR_INVALID, "raw order is not a permutation");
+    seen[(size_t)c] = 1;
+  }
+  ctx->raw.assign(cell_of_raw, cell_of_raw + nraw);
+  return MMF_OK;
+}
+
+int mmf_set_state(mmf_ctx *ctx, const double *U) {
+  if (!ctx || !U) return MMF_ERR_INVALID;
+  std::vector<double> soa;
+  aos_to_soa(ctx, U, soa);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(ctx->dU, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  // the stage buffer starts as a copy so cells never solved hold finite values
+  HIPCHK(ctx, hipMemcpy(ctx->dW, ctx->dU, sizeof(double) * soa.size(), hipMemcpyDeviceToDevice));
+  return MMF_OK;
+}
+
+int mmf_get_state(mmf_ctx *ctx, double *U) {
+  if (!ctx || !U) return MMF_ERR_INVALID;
+  std::vector<double> soa((size_t)(5 * ctx->g.fs));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(soa.data(), ctx->dU, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+  soa_to_aos(ctx, soa, U);
+  return MMF_OK;
+}
+
+int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
+  if (!ctx) return MMF_ERR_INVALID;
+  hipError_t e = mmf::launch_zero_slot(ctx->dctrl, 3, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "zero slot");
+  int rc = launch_stage_timed(ctx, 1, mmf::MODE_RHS_ONLY);
+  if (rc != MMF_OK) return rc;
+  if (rhs) {
+    std::vector<double> soa((size_t)(5 * ctx->g.fs));
+    HIPCHK(ctx, hipMemcpyAsync(soa.data(), ctx->dR, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    soa_to_aos(ctx, soa, rhs);
+  }
+  int r2 = read_ctrl(ctx);
+  if (r2 != MMF_OK) return r2;
+  if (max_eig) {
+    unsigned long long b = ctx->hctrl->eig[3];
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    *max_eig = v;
+  }
+  return MMF_OK;
+}
+
+int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl) {
+  if (!ctx || !(cfl > 0.0)) return fail(ctx, MMF_ERR_INVALID, "cfl must be positive");
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  Ctrl c;
+  std::memset(&c, 0, sizeof(c));
+  c.t = t;
+  c.tmax = tmax;
+  c.cfl = cfl;
+  c.hmin = ctx->g.h;
+  c.neg_p = ctx->hctrl->neg_p;
+  c.neg_rho = ctx->hctrl->neg_rho;
+  *ctx->hctrl = c;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->dctrl, ctx->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MMF_OK;
+}
+
+int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
+  if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    for (int stage = 1; stage <= 3; ++stage) {
+      int rc = launch_stage_timed(ctx, stage, mmf::MODE_STEP);
+      if (rc != MMF_OK) return rc;
+    }
+    hipError_t e = mmf::launch_advance(ctx->dctrl, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "advance kernel");
+  }
+  return MMF_OK;
+}
+
+int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_final) {
+  if (!ctx) return MMF_ERR_INVALID;
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  const int64_t step0 = ctx->hctrl->step;
+  for (;;) {
+    const Ctrl &c = *ctx->hctrl;
+    int64_t done = c.step - step0;
+    if (!(c.t < c.tmax)) break;
+    if (max_steps >= 0 && done >= max_steps) break;
+    int64_t batch = 1;
+    if (c.dt > 0.0) {
+      double rem = std::ceil((c.tmax - c.t) / c.dt);
+      batch = (int64_t)std::min(std::max(rem, 1.0), 256.0);
+    }
+    if (max_steps >= 0) batch = std::min(batch, max_steps - done);
+    rc = mmf_advance(ctx, batch);
+    if (rc != MMF_OK) return rc;
+    rc = read_ctrl(ctx);
+    if (rc != MMF_OK) return rc;
+  }
+  if (steps_done) *steps_done = ctx->hctrl->step - step0;
+  if (t_final) *t_final = ctx->hctrl->t;
+  return MMF_OK;
+}
+
+int mmf_get_time(mmf_ctx *ctx, double *t, double *dt, int64_t *step, double *stage_eig) {
+  if (!ctx) return MMF_ERR_INVALID;
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  const Ctrl &c = *ctx->hctrl;
+  if (t) *t = c.t;
+  if (dt) *dt = c.dt;
+  if (step) *step = c.step;
+  if (stage_eig)
+    for (int s = 0; s < 3; ++s) stage_eig[s] = c.last_eig[s];
+  return MMF_OK;
+}
+
+int mmf_synchronize(mmf_ctx *ctx) {
+  if (!ctx) return MMF_ERR_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MMF_OK;
+}
+
+int mmf_error_norms(mmf_ctx *ctx, const double *rho_exact, double *l1, double *l2) {
+  if (!ctx || !rho_exact) return MMF_ERR_INVALID;
+  const int64_t N = ctx->g.N;
+  std::vector<double> ex((size_t)N);
+  for (int64_t r = 0; r < N; ++r) ex[(size_t)(ctx->raw.empty() ? r : ctx->raw[(size_t)r])] = rho_exact[r];
+  double *dex = nullptr;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMalloc(&dex, sizeof(double) * (size_t)N));
+  hipError_t e = hipMemcpy(dex, ex.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice);
+  double out[2] = {0.0, 0.0};
+  if (e == hipSuccess)
+    e = mmf::launch_error_norms(ctx->g, ctx->dU, dex, ctx->dpartials, ctx->nerr_blocks, ctx->derr, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->derr, sizeof(out), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dex);
+  if (e != hipSuccess) return hip_fail(ctx, e, "error norms");
+  if (l1) *l1 = out[0];
+  if (l2) *l2 = std::sqrt(out[1]);
+  return MMF_OK;
+}
+
+int mmf_get_positivity_counters(mmf_ctx *ctx, int64_t *neg_p, int64_t *neg_rho) {
+  if (!ctx) return MMF_ERR_INVALID;
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  if (neg_p) *neg_p = (int64_t)ctx->hctrl->neg_p;
+  if (neg_rho) *neg_rho = (int64_t)ctx->hctrl->neg_rho;
+  return MMF_OK;
+}
+
+int mmf_reset_positivity_counters(mmf_ctx *ctx) {
+  if (!ctx) return MMF_ERR_INVALID;
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  ctx->hctrl->neg_p = 0;
+  ctx->hctrl->neg_rho = 0;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->dctrl, ctx->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MMF_OK;
+}
+
+int mmf_enable_timing(mmf_ctx *ctx, int32_t on) {
+  if (!ctx) return MMF_ERR_INVALID;
+  int rc = collect_timing(ctx);
+  if (rc != MMF_OK) return rc;
+  ctx->timing = on != 0;
+  ctx->timed_ms = 0.0;
+  ctx->timed_launches = 0;
+  return MMF_OK;
+}
+
+int mmf_read_timing(mmf_ctx *ctx, double *ms, int64_t *launches) {
+  if (!ctx) return MMF_ERR_INVALID;
+  int rc = collect_timing(ctx);
+  if (rc != MMF_OK) return rc;
+  if (ms) *ms = ctx->timed_ms;
+  if (launches) *launches = ctx->timed_launches;
+  return MMF_OK;
+}
+
+// ---- batched point functions ------------------------------------------------
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int batch_run(int64_t count, std::initializer_list<std::pair<const double *, size_t>> ins,
+              std::initializer_list<std::pair<double *, size_t>> outs,
+              const std::function<hipError_t(double **, double **, int64_t)> &launch) {
+  if (count < 0) return MMF_ERR_INVALID;
+  if (count == 0) return MMF_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MMF_ERR_NO_DEVICE;
+  std::vector<DevBuf> bi(ins.size()), bo(outs.size());
+  std::vector<double *> pi, po;
+  size_t i = 0;
+  for (auto &in : ins) {
+    if (!in.first) {
+      pi.push_back(nullptr);
+      ++i;
+      continue;
+    }
+    size_t bytes = sizeof(double) * in.second * (size_t)count;
+    if (hipMalloc(&bi[i].p, bytes) != hipSuccess) return MMF_ERR_OUT_OF_MEMORY;
+    if (hipMemcpy(bi[i].p, in.first, bytes, hipMemcpyHostToDevice) != hipSuccess) return MMF_ERR_HIP;
+    pi.push_back((double *)bi[i].p);
+    ++i;
+  }
+  i = 0;
+  for (auto &o : outs) {
+    size_t bytes = sizeof(double) * o.second * (size_t)count;
+    if (hipMalloc(&bo[i].p, bytes) != hipSuccess) return MMF_ERR_OUT_OF_MEMORY;
+    po.push_back((double *)bo[i].p);
+    ++i;
+  }
+  if (launch(pi.data(), po.data(), count) != hipSuccess) return MMF_ERR_HIP;
+  if (hipDeviceSynchronize() != hipSuccess) return MMF_ERR_HIP;
+  i = 0;
+  for (auto &o : outs) {
+    size_t bytes = sizeof(double) * o.second * (size_t)count;
+    if (hipMemcpy(o.first, bo[i].p, bytes, hipMemcpyDeviceToHost) != hipSuccess) return MMF_ERR_HIP;
+    ++i;
+  }
+  return MMF_OK;
+}
+
+}  // namespace
+
+int mmf_batch_cons2prim(const double *cons, int64_t n, double *prim) {
+  if (!cons || !prim) return MMF_ERR_INVALID;
+  return batch_run(n, {{cons, 5}}, {{prim, 5}}, [](double **i, double **o, int64_t c) {
+    return mmf::launch_batch_cons2prim(i[0], c, o[0], nullptr);
+  });
+}
+
+int mmf_batch_prim2cons(const double *prim, int64_t n, double *cons) {
+  if (!cons || !prim) return MMF_ERR_INVALID;
+  return batch_run(n, {{prim, 5}}, {{cons, 5}}, [](double **i, double **o, int64_t c) {
+    return mmf::launch_batch_prim2cons(i[0], c, o[0], nullptr);
+  });
+}
+
+int mmf_batch_eval_fluxes(const double *cons, const double *prim, const double *nrm, int64_t n, double *flux) {
+  if (!cons || !prim || !nrm || !flux) return MMF_ERR_INVALID;
+  return batch_run(n, {{cons, 5}, {prim, 5}, {nrm, 3}}, {{flux, 5}}, [](double **i, double **o, int64_t c) {
+    return mmf::launch_batch_eval_fluxes(i[0], i[1], i[2], c, o[0], nullptr);
+  });
+}
+
+int mmf_batch_eval_splitting(const double *L, const double *R, const double *nrm, int64_t n, double *F,
+                             double *lam) {
+  if (!L || !R || !nrm || !F || !lam) return MMF_ERR_INVALID;
+  return batch_run(n, {{L, 5}, {R, 5}, {nrm, 3}}, {{F, 5}, {lam, 1}}, [](double **i, double **o, int64_t c) {
+    return mmf::launch_batch_eval_splitting(i[0], i[1], i[2], c, o[0], o[1], nullptr);
+  });
+}
+
+int mmf_batch_eval_bc_values(int32_t bc, const double *nrm, const double *info, const double *cons, int64_t n,
+                             double *out) {
+  if (!nrm || !cons || !out || !valid_bc(bc) || bc == MMF_BC_PERIODIC) return MMF_ERR_INVALID;
+  if (bc == MMF_BC_DIRICHLET && !info) return MMF_ERR_INVALID;
+  return batch_run(n, {{nrm, 3}, {info, 5}, {cons, 5}}, {{out, 5}}, [bc](double **i, double **o, int64_t c) {
+    return mmf::launch_batch_bc_values(bc, i[0], i[1], i[2], c, o[0], nullptr);
+  });
+}
+
+int mmf_calibration_copy(int64_t nbytes, int32_t reps, double *ms) {
+  if (nbytes < 32 || reps < 1 || !ms) return MMF_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MMF_ERR_NO_DEVICE;
+  int64_t n4 = nbytes / 32;
+  DevBuf a, b;
+  if (hipMalloc(&a.p, (size_t)n4 * 32) != hipSuccess) return MMF_ERR_OUT_OF_MEMORY;
+  if (hipMalloc(&b.p, (size_t)n4 * 32) != hipSuccess) return MMF_ERR_OUT_OF_MEMORY;
+  if (hipMemset(a.p, 0, (size_t)n4 * 32) != hipSuccess) return MMF_ERR_HIP;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MMF_ERR_HIP;
+  // warm-up
+  if (mmf::launch_copy((const double4 *)a.p, (double4 *)b.p, n4, nullptr) != hipSuccess) return MMF_ERR_HIP;
+  (void)hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; ++r)
+    if (mmf::launch_copy((const double4 *)a.p, (double4 *)b.p, n4, nullptr) != hipSuccess) return MMF_ERR_HIP;
+  (void)hipEventRecord(e1, nullptr);
+  if (hipEventSynchronize(e1) != hipSuccess) return MMF_ERR_HIP;
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return MMF_OK;
+}
+
+}  // extern "C"

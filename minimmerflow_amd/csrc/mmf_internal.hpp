@@ -1,0 +1,61 @@
+// mmf_internal.hpp -- structures shared by the C-ABI layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mmf {
+
+// Uniform Cartesian grid as the kernels see it (the closed-form
+// MeshGeometricalInfo, mesh_info.hpp:31-105, plus the border BC table).
+struct GridDev {
+  int64_t n[3];        // cells per axis (n[2] == 1 in 2D)
+  int64_t N;           // cells
+  int64_t fs;          // field stride of the SoA state (>= N, multiple of 64)
+  int32_t dim;
+  int32_t order;
+  double h;            // cell size (main.cpp:351-356 minCellSize)
+  double V;            // cell volume h^dim
+  double A;            // face area h^(dim-1)
+  int32_t bc[6];       // x-, x+, y-, y+, z-, z+
+  int32_t periodic[3];
+  double info[6][5];   // Dirichlet primitives per side
+  const uint8_t *flags;  // MMF_CELL_* bits per cell, or nullptr (all fluid+solved)
+};
+
+// Time-control block resident in device memory.
+struct Ctrl {
+  double t, tmax, cfl, hmin;
+  double dt;                     // dt of the last completed step
+  unsigned long long eig[4];     // running stage maxima (bits), slot 3: compute_rhs
+  double last_eig[3];            // stage maxima of the last completed step
+  long long step;
+  unsigned long long neg_p, neg_rho;
+};
+
+enum StageMode { MODE_STEP = 0, MODE_RHS_ONLY = 1 };
+
+// Kernel launchers (mmf_kernels.hip).  All enqueue on 'stream'.
+hipError_t launch_stage(const GridDev &g, int stage, int mode, const double *U, const double *R, const double *W,
+                        double *out_R, double *out_W, double *out_U, Ctrl *ctrl, hipStream_t stream);
+hipError_t launch_advance(Ctrl *ctrl, hipStream_t stream);
+hipError_t launch_zero_slot(Ctrl *ctrl, int slot, hipStream_t stream);
+hipError_t launch_error_norms(const GridDev &g, const double *U, const double *rho_ex, double *partials,
+                              int64_t nblocks, double *out2, hipStream_t stream);
+int64_t error_norm_blocks(int64_t N);
+hipError_t launch_mark_bodies(const GridDev &g, const double *origin, const double *boxes, int nboxes,
+                              const double *spheres, int nspheres, uint8_t *flags, hipStream_t stream);
+hipError_t launch_fill_double(double *p, int64_t n, double v, hipStream_t stream);
+
+// Batched point functions.
+hipError_t launch_batch_cons2prim(const double *c, int64_t n, double *p, hipStream_t s);
+hipError_t launch_batch_prim2cons(const double *p, int64_t n, double *c, hipStream_t s);
+hipError_t launch_batch_eval_fluxes(const double *c, const double *p, const double *nrm, int64_t n, double *f,
+                                    hipStream_t s);
+hipError_t launch_batch_eval_splitting(const double *L, const double *R, const double *nrm, int64_t n, double *F,
+                                       double *lam, hipStream_t s);
+hipError_t launch_batch_bc_values(int bc, const double *nrm, const double *info, const double *c, int64_t n,
+                                  double *out, hipStream_t s);
+hipError_t launch_copy(const double4 *src, double4 *dst, int64_t n4, hipStream_t s);
+
+}  // namespace mmf
