@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256, help="cells per axis")
     ap.add_argument("--order", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--kernel", choices=("auto", "gather", "march"), default="auto")
+    ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4/8/16)")
+    ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -100,6 +103,9 @@ def main():
     N = n ** 3
     pb = mmf.problem("radsod", 3, sod_r0=0.4, domain_origin=[-1.0, -1.0, -1.0], domain_length=2.0)
     s = mmf.Solver.for_problem(pb, n, order=args.order, side_bc=[mmf.BC_FREE_FLOW] * 6)
+    kind = {"auto": mmf.KERNEL_AUTO, "gather": mmf.KERNEL_GATHER, "march": mmf.KERNEL_MARCH}[args.kernel]
+    s.set_kernel(kind, args.rows, args.chunk)
+    kinfo = s.kernel()
     t_setup = time.perf_counter()
     s.set_initial_state(pb)
     s.set_time(0.0, 0.25, 0.5)
@@ -167,7 +173,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": load_traffic(workload),
-            "kernel": "k_stage (fused flux + RK stage), mean of stages 1-3",
+            "kernel": "%s (fused flux + RK stage), mean of stages 1-3" % (
+                "k_march tile 64x%d, %d planes/block" % (kinfo[1], kinfo[2]) if kinfo[0] == mmf.KERNEL_MARCH
+                else "k_stage (per-cell gather)"),
             "algorithmic_bytes_per_launch": BYTES_PER_CELL_STAGE * N,
             "avg_launch_ms": avg_ms,
         },

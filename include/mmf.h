@@ -82,6 +82,16 @@ const char *mmf_status_string(int status);
 /* cells; interfaces as the reference counts them (border faces included) */
 int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *ninterfaces);
 
+/* ---- stage-kernel selection ------------------------------------------------ */
+enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2 };
+/* AUTO: the marching LDS-tile kernel wherever it applies (order 1), else the
+ * per-cell gather kernel.  tile_rows (3D: 4, 8, 16) and chunk (planes per
+ * block along the last axis) shape the marching kernel; <= 0 keeps the
+ * default.  Every variant gives bit-identical results. */
+int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk);
+/* kind actually used for the next launches, and the marching shape */
+int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk);
+
 /* ---- body / flags (body::isPointFluid + main.cpp:222-237, 251-277) ------- */
 /* fluid: [ncells] 0/1 or NULL (all fluid); solved: NULL -> solved = fluid */
 int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved);

@@ -23,6 +23,7 @@ NF = 5
 OK, ERR_INVALID, ERR_UNSUPPORTED_ORDER, ERR_HIP, ERR_NO_DEVICE, ERR_OOM, ERR_STATE = range(7)
 BC_NONE, BC_FREE_FLOW, BC_REFLECTING, BC_WALL, BC_DIRICHLET, BC_PERIODIC = -1, 0, 1, 2, 3, 4
 CELL_FLUID, CELL_SOLVED = 1, 2
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_MARCH = 0, 1, 2
 (VORTEX_XY, VORTEX_ZX, VORTEX_YZ, RADSOD, SOD_X, SOD_Y, SOD_Z, FFSTEP, FREESTREAM) = range(9)
 PROBLEM_KEYS = {"vortex_xy": VORTEX_XY, "vortex_zx": VORTEX_ZX, "vortex_yz": VORTEX_YZ,
                 "radsod": RADSOD, "sod3d_x": SOD_X, "sod3d_y": SOD_Y, "sod3d_z": SOD_Z,
@@ -50,6 +51,16 @@ class Problem(C.Structure):
 
 
 _lib = None
+# kernel variant applied to every new Solver: (kind, tile_rows, chunk)
+_default_kernel = (KERNEL_AUTO, 0, 0)
+
+
+def set_default_kernel(kind=KERNEL_AUTO, tile_rows=0, chunk=0):
+    """Stage-kernel variant for Solvers created afterwards (AUTO restores the default).
+    A variant that does not apply to a grid (the marching kernel at order 2)
+    leaves that grid on its automatic choice."""
+    global _default_kernel
+    _default_kernel = (int(kind), int(tile_rows), int(chunk))
 
 
 def _status_string(s):
@@ -107,6 +118,8 @@ def lib():
         "mmf_error_vs_exact": ([vp, C.POINTER(Problem), C.c_double, dp, dp], C.c_int),
         "mmf_get_centroids": ([vp, dp], C.c_int),
         "mmf_calibration_copy": ([C.c_int64, C.c_int32, dp], C.c_int),
+        "mmf_set_kernel": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "mmf_get_kernel": ([vp, i32p, i32p, i32p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -250,6 +263,10 @@ class Solver:
         nc, nf = C.c_int64(), C.c_int64()
         _check(lib().mmf_get_sizes(self.ctx, C.byref(nc), C.byref(nf)))
         self.ncells, self.ninterfaces = nc.value, nf.value
+        if _default_kernel != (KERNEL_AUTO, 0, 0):
+            rc = lib().mmf_set_kernel(self.ctx, *_default_kernel)
+            if rc not in (OK, ERR_INVALID):
+                _check(rc, self.ctx)
 
     @classmethod
     def for_problem(cls, pb, ncells, order=1, side_bc=None, origin=None, length=None):
@@ -309,6 +326,15 @@ class Solver:
         a = np.ascontiguousarray(cell_of_raw, dtype=np.int64)
         self._raw = a
         self._c(lib().mmf_set_raw_order(self.ctx, a.ctypes.data_as(C.POINTER(C.c_int64)), a.size))
+
+    # stage-kernel variant
+    def set_kernel(self, kind=KERNEL_AUTO, tile_rows=0, chunk=0):
+        self._c(lib().mmf_set_kernel(self.ctx, int(kind), int(tile_rows), int(chunk)))
+
+    def kernel(self):
+        k, ty, ch = C.c_int32(), C.c_int32(), C.c_int32()
+        self._c(lib().mmf_get_kernel(self.ctx, C.byref(k), C.byref(ty), C.byref(ch)))
+        return k.value, ty.value, ch.value
 
     # bodies / flags
     def set_cell_flags(self, fluid, solved=None):

@@ -34,6 +34,8 @@ struct mmf_ctx {
   int64_t nerr_blocks = 0;
   double *derr = nullptr;
   std::vector<int64_t> raw;  // cell_of_raw, empty when identity
+  int kernel = mmf::KERNEL_AUTO;
+  mmf::MarchShape shape{8, 32};
   std::string err;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -96,8 +98,11 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
     HIPCHK(ctx, hipEventCreate(&e1));
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
   }
-  hipError_t e = mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
-                                   ctx->dctrl, ctx->stream);
+  const bool march = ctx->kernel != mmf::KERNEL_GATHER && mmf::march_shape_supported(ctx->g, ctx->shape);
+  hipError_t e = march ? mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR,
+                                                 ctx->dW, ctx->dU, ctx->dctrl, ctx->stream)
+                       : mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
+                                           ctx->dctrl, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "stage kernel launch");
   if (ctx->timing) {
     HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
@@ -180,6 +185,12 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
   }
   for (int a = 0; a < 3; ++a) g.periodic[a] = (a < d->dim) && d->side_bc[2 * a] == MMF_BC_PERIODIC;
   g.flags = nullptr;
+  {
+    int e2 = 0;
+    g.pow2V = std::frexp(g.V, &e2) == 0.5 ? 1 : 0;
+    g.invV = 1.0 / g.V;
+  }
+  ctx->shape = mmf::default_march_shape(g);
 
   auto cleanup = [&](int code) {
     mmf_destroy(ctx);
@@ -225,6 +236,28 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->derr) (void)hipFree(ctx->derr);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return MMF_OK;
+}
+
+int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk) {
+  if (!ctx) return MMF_ERR_INVALID;
+  if (kind < mmf::KERNEL_AUTO || kind > mmf::KERNEL_MARCH) return fail(ctx, MMF_ERR_INVALID, "unknown kernel kind");
+  mmf::MarchShape sh = mmf::default_march_shape(ctx->g);
+  if (tile_rows > 0) sh.ty = tile_rows;
+  if (chunk > 0) sh.chunk = chunk;
+  if (kind == mmf::KERNEL_MARCH && !mmf::march_shape_supported(ctx->g, sh))
+    return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1 only; tile rows 4/8/16 in 3D, 1 in 2D");
+  ctx->kernel = kind;
+  ctx->shape = sh;
+  return MMF_OK;
+}
+
+int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk) {
+  if (!ctx) return MMF_ERR_INVALID;
+  const bool march = ctx->kernel != mmf::KERNEL_GATHER && mmf::march_shape_supported(ctx->g, ctx->shape);
+  if (kind) *kind = march ? mmf::KERNEL_MARCH : mmf::KERNEL_GATHER;
+  if (tile_rows) *tile_rows = ctx->shape.ty;
+  if (chunk) *chunk = ctx->shape.chunk;
   return MMF_OK;
 }
 

@@ -21,6 +21,17 @@ struct GridDev {
   int32_t periodic[3];
   double info[6][5];   // Dirichlet primitives per side
   const uint8_t *flags;  // MMF_CELL_* bits per cell, or nullptr (all fluid+solved)
+  double invV;         // 1/V, used only when V is a power of two (x/V == x*invV exactly)
+  int32_t pow2V;
+};
+
+// Stage-kernel variants.
+enum KernelKind { KERNEL_AUTO = 0, KERNEL_GATHER = 1, KERNEL_MARCH = 2 };
+
+// Launch shape of the marching kernel (tile rows and march chunk).
+struct MarchShape {
+  int ty;     // tile rows (3D): 4, 8 or 16; 1 in 2D
+  int chunk;  // planes per block along the march axis
 };
 
 // Time-control block resident in device memory.
@@ -38,6 +49,11 @@ enum StageMode { MODE_STEP = 0, MODE_RHS_ONLY = 1 };
 // Kernel launchers (mmf_kernels.hip).  All enqueue on 'stream'.
 hipError_t launch_stage(const GridDev &g, int stage, int mode, const double *U, const double *R, const double *W,
                         double *out_R, double *out_W, double *out_U, Ctrl *ctrl, hipStream_t stream);
+hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int stage, int mode, const double *U,
+                              const double *R, const double *W, double *out_R, double *out_W, double *out_U,
+                              Ctrl *ctrl, hipStream_t stream);
+bool march_shape_supported(const GridDev &g, const MarchShape &shape);
+MarchShape default_march_shape(const GridDev &g);
 hipError_t launch_advance(Ctrl *ctrl, hipStream_t stream);
 hipError_t launch_zero_slot(Ctrl *ctrl, int slot, hipStream_t stream);
 hipError_t launch_error_norms(const GridDev &g, const double *U, const double *rho_ex, double *partials,

@@ -16,103 +16,11 @@
 
 #include "mmf_device.hpp"
 #include "mmf_internal.hpp"
+#include "mmf_stage.hpp"
 
 namespace mmf {
 
 constexpr int BLOCK = 256;
-
-// ---------------------------------------------------------------------------
-// block max of non-negative doubles (as u64 bits), one atomic per block
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void block_max_to(unsigned long long *dst, double v) {
-  __shared__ unsigned long long red[BLOCK / 64];
-  unsigned long long b = dbits(v);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(b, off, 64);
-    b = (o > b) ? o : b;
-  }
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = b;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = red[0];
-#pragma unroll
-    for (int w = 1; w < BLOCK / 64; ++w) m = (red[w] > m) ? red[w] : m;
-    if (m) atomicMax(dst, m);
-  }
-}
-
-// dt rule of main.cpp:398-402, evaluated identically by every thread.
-__device__ __forceinline__ double step_dt(const Ctrl *c) {
-  double me = __longlong_as_double((long long)c->eig[0]);
-  double dt = 0.9 * c->cfl * c->hmin / me;
-  if (c->t + dt > c->tmax) dt = c->tmax - c->t;
-  return dt;
-}
-
-// Stage input state of one cell.
-template <int STAGE>
-struct StageIn {
-  const double *X;   // U (stage 1) or W (stage 3)
-  const double *U;   // stage 2
-  const double *R;   // stage 2
-  int64_t fs;
-  double dt, V;
-  __device__ __forceinline__ void load(int64_t idx, double *c) const {
-    if (STAGE == 2) {
-#pragma unroll
-      for (int k = 0; k < NF; ++k) c[k] = U[k * fs + idx] + dt * R[k * fs + idx] / V;  // main.cpp:421
-    } else {
-#pragma unroll
-      for (int k = 0; k < NF; ++k) c[k] = X[k * fs + idx];
-    }
-  }
-};
-
-__device__ __forceinline__ bool neighbour(const GridDev &g, const int64_t *q, int a, int dir, int64_t &nb) {
-  int64_t v = q[a] + dir;
-  if (v < 0 || v >= g.n[a]) {
-    if (!g.periodic[a]) return false;
-    v = (v + g.n[a]) % g.n[a];
-  }
-  int64_t r[3] = {q[0], q[1], q[2]};
-  r[a] = v;
-  nb = r[0] + g.n[0] * (r[1] + g.n[1] * r[2]);
-  return true;
-}
-
-template <bool MASK>
-__device__ __forceinline__ uint8_t cell_flags(const GridDev &g, int64_t idx) {
-  return MASK ? g.flags[idx] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
-}
-
-__device__ __forceinline__ double minmod(double a, double b) {
-  if (a > 0.0 && b > 0.0) return (a < b) ? a : b;
-  if (a < 0.0 && b < 0.0) return (a > b) ? a : b;
-  return 0.0;
-}
-
-// Border virtual state (euler.cpp:261-376) from the fluid-side face value.
-__device__ __forceinline__ void border_state(const GridDev &g, int side, int a, const double *fluid, double *out) {
-  int bc = g.bc[side];
-  if (bc == BC_DIRICHLET) {
-    prim2cons(g.info[side], out);
-  } else if (bc == BC_REFLECTING || bc == BC_WALL) {
-    reflect_axis(fluid, a, out);
-  } else {
-#pragma unroll
-    for (int k = 0; k < NF; ++k) out[k] = fluid[k];
-  }
-}
-
-// Count negative p / rho of one face's two sides (euler.cpp:94-101).
-__device__ __forceinline__ void count_neg(const Tup &L, const Tup &R, Ctrl *ctrl) {
-  int np = (L.p < 0.) + (R.p < 0.);
-  int nr = (L.c[0] < 0.) + (R.c[0] < 0.);
-  if (np) atomicAdd(&ctrl->neg_p, (unsigned long long)np);
-  if (nr) atomicAdd(&ctrl->neg_rho, (unsigned long long)nr);
-}
 
 // ---------------------------------------------------------------------------
 // stage kernel
@@ -133,7 +41,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
     const uint8_t fc = cell_flags<MASK>(g, idx);
     const bool solved = fc & CELL_SOLVED;
     double Xc[NF];
-    in.load(idx, Xc);
+    in.load(g, idx, Xc);
 
     if (solved) {
       double rhs[NF] = {0., 0., 0., 0., 0.};  // euler.cpp:142-148
@@ -159,27 +67,27 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
           for (int k = 0; k < NF; ++k) Rm[k] = Xc[k], Lp[k] = Xc[k];
           if (!em1) border_state(g, 2 * a, a, Xc, Lm);
           else if (wall_m) reflect_axis(Xc, a, Lm);
-          else in.load(nm1, Lm);
+          else in.load(g, nm1, Lm);
           if (!ep1) border_state(g, 2 * a + 1, a, Xc, Rp);
           else if (wall_p) reflect_axis(Xc, a, Rp);
-          else in.load(np1, Rp);
+          else in.load(g, np1, Rp);
         } else {
           // limited linear: slopes of cells -1, 0, +1 along a
           double Xm1[NF], Xp1[NF], Xm2[NF], Xp2[NF];
-          if (em1) in.load(nm1, Xm1);
-          if (ep1) in.load(np1, Xp1);
+          if (em1) in.load(g, nm1, Xm1);
+          if (ep1) in.load(g, np1, Xp1);
           int64_t q1[3] = {q[0], q[1], q[2]};
           bool em2 = false, ep2 = false;
           uint8_t fm2 = 0, fp2 = 0;
           if (em1) {
             q1[a] = (q[a] - 1 + g.n[a]) % g.n[a];
             em2 = neighbour(g, q1, a, -1, nm2);
-            if (em2) { fm2 = cell_flags<MASK>(g, nm2); in.load(nm2, Xm2); }
+            if (em2) { fm2 = cell_flags<MASK>(g, nm2); in.load(g, nm2, Xm2); }
           }
           if (ep1) {
             q1[a] = (q[a] + 1) % g.n[a];
             ep2 = neighbour(g, q1, a, +1, np2);
-            if (ep2) { fp2 = cell_flags<MASK>(g, np2); in.load(np2, Xp2); }
+            if (ep2) { fp2 = cell_flags<MASK>(g, np2); in.load(g, np2, Xp2); }
           }
           const bool okc_m = em1 && (fm1 & CELL_FLUID);  // this cell's lower difference
           const bool okc_p = ep1 && (fp1 & CELL_FLUID);
@@ -263,23 +171,10 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
       }
 
       const int64_t fs = g.fs;
-      if (STAGE == 1) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) out[k * fs + idx] = rhs[k];
-      } else if (STAGE == 2) {
-        const double dt = in.dt, V = g.V;
-#pragma unroll
-        for (int k = 0; k < NF; ++k) {
-          double u = U[k * fs + idx];
-          out[k * fs + idx] = 0.75 * u + 0.25 * (Xc[k] + dt * rhs[k] / V);  // main.cpp:457
-        }
-      } else {
-        const double dt = in.dt, V = g.V;
-#pragma unroll
-        for (int k = 0; k < NF; ++k) {
-          double u = U[k * fs + idx];
-          out[k * fs + idx] = (1. / 3) * u + (2. / 3) * (Xc[k] + dt * rhs[k] / V);  // main.cpp:493
-        }
+      for (int k = 0; k < NF; ++k) {
+        double u = (STAGE == 1) ? 0.0 : U[k * fs + idx];
+        out[k * fs + idx] = rk_update<STAGE>(g, u, Xc[k], rhs[k], in.dt);
       }
     } else {
       const int64_t fs = g.fs;
@@ -292,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
       }
     }
   }
-  block_max_to(&ctrl->eig[slot], lam_max);
+  block_max_to<BLOCK>(&ctrl->eig[slot], lam_max, threadIdx.x);
 }
 
 // t += dt after stage 3 (main.cpp:505-506); resets the stage maxima.
@@ -320,7 +215,6 @@ static hipError_t launch_stage_t(const GridDev &g, int mode, const double *U, co
                                  double *out_R, double *out_W, double *out_U, Ctrl *ctrl, hipStream_t stream) {
   StageIn<STAGE> in;
   in.fs = g.fs;
-  in.V = g.V;
   in.dt = 0.0;
   in.U = U;
   in.R = R;

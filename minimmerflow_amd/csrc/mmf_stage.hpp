@@ -1,0 +1,118 @@
+// mmf_stage.hpp -- device helpers shared by the stage kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mmf_device.hpp"
+#include "mmf_internal.hpp"
+
+namespace mmf {
+
+// x / V of the RK update (main.cpp:421,457,493).  When V is a power of two the
+// product with 1/V is the same double, and costs one multiply instead of a
+// division sequence.
+__device__ __forceinline__ double div_vol(const GridDev &g, double x) { return g.pow2V ? x * g.invV : x / g.V; }
+
+// dt rule of main.cpp:398-402, evaluated identically by every thread.
+__device__ __forceinline__ double step_dt(const Ctrl *c) {
+  double me = __longlong_as_double((long long)c->eig[0]);
+  double dt = 0.9 * c->cfl * c->hmin / me;
+  if (c->t + dt > c->tmax) dt = c->tmax - c->t;
+  return dt;
+}
+
+// Stage input state of one cell: U (stage 1), W1 = U + dt R / V rebuilt on the
+// fly (stage 2), W (stage 3).
+template <int STAGE>
+struct StageIn {
+  const double *X;
+  const double *U;
+  const double *R;
+  int64_t fs;
+  double dt;
+  __device__ __forceinline__ void load(const GridDev &g, int64_t idx, double *c) const {
+    if (STAGE == 2) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) c[k] = U[k * fs + idx] + div_vol(g, dt * R[k * fs + idx]);  // main.cpp:421
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) c[k] = X[k * fs + idx];
+    }
+  }
+};
+
+// Block max of non-negative doubles (compared as u64 bits), one atomic per block.
+template <int NT>
+__device__ __forceinline__ void block_max_to(unsigned long long *dst, double v, int tid) {
+  __shared__ unsigned long long red[(NT + 63) / 64];
+  unsigned long long b = dbits(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off, 64);
+    b = (o > b) ? o : b;
+  }
+  const int lane = tid & 63, wid = tid >> 6;
+  if (lane == 0) red[wid] = b;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long m = red[0];
+#pragma unroll
+    for (int w = 1; w < (NT + 63) / 64; ++w) m = (red[w] > m) ? red[w] : m;
+    if (m) atomicMax(dst, m);
+  }
+}
+
+// Neighbour of cell q along axis a (dir +-1); false outside a non-periodic border.
+__device__ __forceinline__ bool neighbour(const GridDev &g, const int64_t *q, int a, int dir, int64_t &nb) {
+  int64_t v = q[a] + dir;
+  if (v < 0 || v >= g.n[a]) {
+    if (!g.periodic[a]) return false;
+    v = (v + g.n[a]) % g.n[a];
+  }
+  int64_t r[3] = {q[0], q[1], q[2]};
+  r[a] = v;
+  nb = r[0] + g.n[0] * (r[1] + g.n[1] * r[2]);
+  return true;
+}
+
+template <bool MASK>
+__device__ __forceinline__ uint8_t cell_flags(const GridDev &g, int64_t idx) {
+  return MASK ? g.flags[idx] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
+}
+
+__device__ __forceinline__ double minmod(double a, double b) {
+  if (a > 0.0 && b > 0.0) return (a < b) ? a : b;
+  if (a < 0.0 && b < 0.0) return (a > b) ? a : b;
+  return 0.0;
+}
+
+// Border virtual state (euler.cpp:261-376) from the fluid-side face value.
+__device__ __forceinline__ void border_state(const GridDev &g, int side, int a, const double *fluid, double *out) {
+  int bc = g.bc[side];
+  if (bc == BC_DIRICHLET) {
+    prim2cons(g.info[side], out);
+  } else if (bc == BC_REFLECTING || bc == BC_WALL) {
+    reflect_axis(fluid, a, out);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) out[k] = fluid[k];
+  }
+}
+
+// Count negative p / rho of one face's two sides (euler.cpp:94-101).
+__device__ __forceinline__ void count_neg(const Tup &L, const Tup &R, Ctrl *ctrl) {
+  int np = (L.p < 0.) + (R.p < 0.);
+  int nr = (L.c[0] < 0.) + (R.c[0] < 0.);
+  if (np) atomicAdd(&ctrl->neg_p, (unsigned long long)np);
+  if (nr) atomicAdd(&ctrl->neg_rho, (unsigned long long)nr);
+}
+
+// RK stage update of one field (main.cpp:421,457,493).
+template <int STAGE>
+__device__ __forceinline__ double rk_update(const GridDev &g, double u, double x, double rhs, double dt) {
+  if (STAGE == 1) return rhs;
+  if (STAGE == 2) return 0.75 * u + 0.25 * (x + div_vol(g, dt * rhs));
+  return (1. / 3) * u + (2. / 3) * (x + div_vol(g, dt * rhs));
+}
+
+}  // namespace mmf
