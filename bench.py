@@ -90,7 +90,16 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import minimmerflow_amd as mmf
+        ms = mmf.calibration_copy(args.copy_calibration, reps=5)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "ms_per_copy": ms,
+                          "GBs_read_plus_write": 2 * args.copy_calibration / (ms * 1e-3) / 1e9}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "

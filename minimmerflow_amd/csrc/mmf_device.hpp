@@ -155,16 +155,28 @@ __device__ __forceinline__ void side_flux(const Tup &t, int a, double *f) {
   f[4] = un * t.H;
 }
 
-// euler.cpp:42-73 along +e_a.
+// euler.cpp:42-73 along +e_a.  The per-side fluxes (side_flux) are formed
+// component by component so that few of them are live at a time; the
+// operations and their order are those of side_flux + the splitting sum.
 __device__ __forceinline__ double face_flux(const Tup &L, const Tup &R, int a, double *F) {
-  double fL[NF], fR[NF];
-  side_flux(L, a, fL);
-  side_flux(R, a, fR);
-  double lambdaL = fabs(L.vel[a]) + L.a;
-  double lambdaR = fabs(R.vel[a]) + R.a;
-  double lam = (lambdaR < lambdaL) ? lambdaL : lambdaR;
+  const double unL = L.vel[a], unR = R.vel[a];
+  const double lambdaL = fabs(unL) + L.a;
+  const double lambdaR = fabs(unR) + R.a;
+  const double lam = (lambdaR < lambdaL) ? lambdaL : lambdaR;
+  const double mfL = L.c[FID_RHO] * unL;
+  const double mfR = R.c[FID_RHO] * unR;
+  F[0] = 0.5 * ((mfR + mfL) - lam * (R.c[0] - L.c[0]));
 #pragma unroll
-  for (int k = 0; k < NF; ++k) F[k] = 0.5 * ((fR[k] + fL[k]) - lam * (R.c[k] - L.c[k]));
+  for (int b = 0; b < 3; ++b) {
+    double fLb = mfL * L.vel[b];
+    double fRb = mfR * R.vel[b];
+    if (b == a) {
+      fLb = fLb + L.p;
+      fRb = fRb + R.p;
+    }
+    F[1 + b] = 0.5 * ((fRb + fLb) - lam * (R.c[1 + b] - L.c[1 + b]));
+  }
+  F[4] = 0.5 * ((unR * R.H + unL * L.H) - lam * (R.c[4] - L.c[4]));
   return lam;
 }
 

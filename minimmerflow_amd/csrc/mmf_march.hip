@@ -2,16 +2,16 @@
 //
 // A block owns a tile of 64 (x, one wave per row) by TY (y) cells and marches
 // along the last axis (z in 3D, y in 2D) over a chunk of planes.  Per plane:
-//   1. every thread loads its cell (coalesced SoA rows of 512 B per field),
-//      computes the cell's flux tuple once (cons2prim, sound speed, total
-//      enthalpy: every division and the sqrt of the reference's per-face
-//      evaluation, done once per cell) and stores it in LDS; a few threads do
-//      the same for the tile's one-cell halo ring;
-//   2. the x and y faces of each cell read the neighbour tuples from LDS;
-//      the face between this plane and the previous one uses the previous
-//      tuple kept in registers, so each z face is evaluated once;
-//   3. the previous cell's residual is complete: its RK stage update is
-//      applied and written.
+//   1. every thread turns its cell's state (loaded one plane ahead, so the
+//      HBM latency hides behind the previous plane's work) into the cell's
+//      flux tuple -- cons2prim, sound speed, total enthalpy: every division
+//      and the sqrt of the reference's per-face evaluation, done once per
+//      cell -- and stores it in LDS; the first threads do the same for the
+//      tile's one-cell halo ring;
+//   2. the face between this plane and the previous one uses the previous
+//      tuple kept in registers, so each march-axis face is evaluated once; it
+//      completes the previous cell, whose RK stage update is written;
+//   3. the x and y faces of each cell read the neighbour tuples from LDS.
 // Residual contributions are accumulated per cell in the reference's face
 // order (x lower, x upper, y lower, y upper, z lower, z upper), so the result
 // is bit-identical to the gather kernel and to the oracle; no float atomics.
@@ -54,92 +54,53 @@ __device__ __forceinline__ void tup_load(const double *lds, int slot, Tup &t) {
   t.H = lds[10 * PL + slot];
 }
 
-// Virtual state across a border side or a wall, as a tuple.
-__device__ __forceinline__ void virtual_tup(const GridDev &g, bool wall, int side, int a, const Tup &fluid, Tup &out) {
+// Flux of a face whose other side is a virtual state (border side 'side' or
+// a wall): the virtual state is built from the fluid tuple F.  fluid_lower:
+// the fluid cell is the lower (L) side of the face.
+__device__ __forceinline__ double virtual_face(const GridDev &g, bool wall, int side, int a, const Tup &Fl,
+                                            bool fluid_lower, double *F, Ctrl *ctrl, bool count) {
+  Tup V;
   if (!wall && g.bc[side] == BC_FREE_FLOW) {  // euler.cpp:298-308: a copy
-    out = fluid;
-    return;
+    V = Fl;
+  } else {
+    double v[NF];
+    if (wall) reflect_axis(Fl.c, a, v);  // euler.cpp:347-356
+    else border_state(g, side, a, Fl.c, v);
+    make_tup(v, V);
   }
-  double v[NF];
-  if (wall) reflect_axis(fluid.c, a, v);  // euler.cpp:347-356
-  else border_state(g, side, a, fluid.c, v);
-  make_tup(v, out);
+  double lam;
+  if (fluid_lower) {
+    lam = face_flux(Fl, V, a, F);
+    if (count) count_neg(Fl, V, ctrl);
+  } else {
+    lam = face_flux(V, Fl, a, F);
+    if (count) count_neg(V, Fl, ctrl);
+  }
+  return lam;
 }
 
-// Lower and upper faces of this cell along an in-plane axis a; the
-// neighbour tuples come from LDS slots sm / sp.  Adds the two contributions
-// to acc in reference order.
-template <int PL, bool MASK>
-__device__ __forceinline__ void inplane_faces(const GridDev &g, const double *lds, const uint8_t *lfl, int a,
-                                              bool has_m, bool has_p, int slot_m, int slot_p, bool wrap_first,
-                                              const Tup &Tc, uint8_t fc, double *acc, double &lam_max,
-                                              Ctrl *ctrl, bool count) {
-  double Fm[NF], Fp[NF];
-  const uint8_t fm = has_m ? (MASK ? lfl[slot_m] : (uint8_t)(CELL_FLUID | CELL_SOLVED)) : 0;
-  const uint8_t fp = has_p ? (MASK ? lfl[slot_p] : (uint8_t)(CELL_FLUID | CELL_SOLVED)) : 0;
-  {
-    const bool wall = MASK && has_m && ((fm ^ fc) & CELL_FLUID);
-    Tup L;
-    if (has_m && !wall) tup_load<PL>(lds, slot_m, L);
-    else virtual_tup(g, wall, 2 * a, a, Tc, L);
-    double lm = face_flux(L, Tc, a, Fm);
-    lam_max = (lm < lam_max) ? lam_max : lm;
-    const bool other_counts = has_m && !wall && (fm & CELL_SOLVED);
-    if (count && !other_counts) count_neg(L, Tc, ctrl);
-  }
-  {
-    const bool wall = MASK && has_p && ((fp ^ fc) & CELL_FLUID);
-    Tup R;
-    if (has_p && !wall) tup_load<PL>(lds, slot_p, R);
-    else virtual_tup(g, wall, 2 * a + 1, a, Tc, R);
-    double lp = face_flux(Tc, R, a, Fp);
-    lam_max = (lp < lam_max) ? lam_max : lp;
-    if (count) count_neg(Tc, R, ctrl);
-  }
-  const double A = g.A;
-  if (wrap_first) {
+template <int STAGE>
+struct Prefetch {
+  double a[NF];  // X (stages 1, 3) or U (stage 2)
+  double b[NF];  // R (stage 2)
+  __device__ __forceinline__ void issue(const StageIn<STAGE> &in, int64_t fs, int64_t idx) {
+    if (STAGE == 2) {
 #pragma unroll
-    for (int k = 0; k < NF; ++k) { acc[k] -= A * Fp[k]; acc[k] += A * Fm[k]; }
-  } else {
-#pragma unroll
-    for (int k = 0; k < NF; ++k) { acc[k] += A * Fm[k]; acc[k] -= A * Fp[k]; }
-  }
-}
-
-// Face between the previous (L) and the current (R) cell along the march
-// axis M.  Returns false when the face is inactive (neither side solved).
-template <bool MASK>
-__device__ __forceinline__ bool march_face(const GridDev &g, int M, bool have_l, bool have_r, const Tup &Tl,
-                                           uint8_t fl, const Tup &Tr, uint8_t fr, double *F, double &lam_max,
-                                           Ctrl *ctrl, bool count) {
-  Tup L, R;
-  if (!have_l) {  // lower border of the march axis: virtual from R
-    if (!(fr & CELL_SOLVED)) return false;
-    virtual_tup(g, false, 2 * M, M, Tr, L);
-    R = Tr;
-  } else if (!have_r) {  // upper border
-    if (!(fl & CELL_SOLVED)) return false;
-    L = Tl;
-    virtual_tup(g, false, 2 * M + 1, M, Tl, R);
-  } else {
-    if (!((fl | fr) & CELL_SOLVED)) return false;
-    const bool wall = MASK && ((fl ^ fr) & CELL_FLUID);
-    if (wall && (fr & CELL_FLUID)) {
-      virtual_tup(g, true, 2 * M, M, Tr, L);
-      R = Tr;
-    } else if (wall) {
-      L = Tl;
-      virtual_tup(g, true, 2 * M + 1, M, Tl, R);
+      for (int k = 0; k < NF; ++k) {
+        a[k] = in.U[k * fs + idx];
+        b[k] = in.R[k * fs + idx];
+      }
     } else {
-      L = Tl;
-      R = Tr;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) a[k] = in.X[k * fs + idx];
     }
   }
-  double lam = face_flux(L, R, M, F);
-  lam_max = (lam < lam_max) ? lam_max : lam;
-  if (count) count_neg(L, R, ctrl);
-  return true;
-}
+  // the stage input state of the cell (main.cpp:421 for stage 2)
+  __device__ __forceinline__ void state(const GridDev &g, double dt, double *X) const {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) X[k] = (STAGE == 2) ? a[k] + div_vol(g, dt * b[k]) : a[k];
+  }
+};
 
 }  // namespace
 
@@ -150,13 +111,15 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   constexpr int M = DIM - 1;  // march axis
   constexpr int RW = TX + 2;
   constexpr int PL = (TY + 2) * RW;
-  __shared__ double lds[NTUP * PL];
-  __shared__ uint8_t lfl[MASK ? PL : 1];
+  // two tuple planes (one barrier per plane) where they fit in LDS
+  constexpr int NBUF = (TY <= 8) ? 2 : 1;
+  __shared__ double lds[NBUF * NTUP * PL];
+  __shared__ uint8_t lfl[MASK ? NBUF * PL : 1];
 
   const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
-  if (STAGE >= 2) in.dt = step_dt(ctrl);
-  const double dt = in.dt;
+  const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+  in.dt = dt;  // the chunk-boundary loads (in.load) rebuild W1 with it
 
   // XCD-aware tile numbering
   const int nblk = gridDim.x;
@@ -186,7 +149,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const int myslot = (row + 1) * RW + lane + 1;
 
   // halo ring cell handled by this thread (one per plane), if any
-  int64_t hcol = -1;
+  int64_t hcol = 0;
   int hslot = -1;
   {
     const int H = (DIM == 3) ? 2 * w + 2 * hgt : 2;
@@ -224,148 +187,244 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const bool has_ym = DIM == 3 && valid && (gy > 0 || g.periodic[1]);
   const bool has_yp = DIM == 3 && valid && (gy + 1 < ny || g.periodic[1]);
   const bool wrap_y = DIM == 3 && g.periodic[1] && gy == ny - 1;
-  const bool count = true;
+  const bool top_wrap_here = g.periodic[M] && m1 == nm;  // this chunk holds the periodic top cell
+  const double A = g.A;
 
   double lam_max = 0.0;
-  Tup Tp;                                   // previous plane's tuple (this column)
+  Tup Tp;                                    // previous plane's tuple (this column)
   uint8_t fprev = CELL_FLUID | CELL_SOLVED;
-  double pxy[NF], zlo[NF], Up[NF];
+  double part[NF];                           // residual of the previous cell so far
 #pragma unroll
-  for (int k = 0; k < NF; ++k) pxy[k] = zlo[k] = Up[k] = 0.0;
+  for (int k = 0; k < NF; ++k) part[k] = 0.0;
 
   // prologue: the plane below the chunk
-  int64_t mprev = m0 - 1;
-  bool have_prev = true;
-  if (mprev < 0) {
-    if (g.periodic[M]) mprev += nm;
-    else have_prev = false;
+  {
+    int64_t mprev = m0 - 1;
+    bool have_prev = true;
+    if (mprev < 0) {
+      if (g.periodic[M]) mprev += nm;
+      else have_prev = false;
+    }
+    if (valid && have_prev) {
+      double X[NF];
+      in.load(g, col + mprev * sm, X);
+      make_tup(X, Tp);
+      fprev = cell_flags<MASK>(g, col + mprev * sm);
+    }
   }
-  if (valid && have_prev) {
-    double X[NF];
-    in.load(g, col + mprev * sm, X);
-    make_tup(X, Tp);
-    fprev = cell_flags<MASK>(g, col + mprev * sm);
-  }
+  const bool have_prev0 = (m0 > 0) || g.periodic[M];
 
+  Prefetch<STAGE> pf, ph;
+  if (valid) pf.issue(in, fs, col + m0 * sm);
+  if (hslot >= 0) ph.issue(in, fs, hcol + m0 * sm);
+
+  // U of the previous cell for its RK update (stages 2 and 3), read one plane ahead
+  double Uf[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) Uf[k] = 0.0;
+
+  int buf = 0;
   for (int64_t m = m0; m < m1; ++m) {
     const int64_t idx = col + m * sm;
+    double *L = lds + buf * (NTUP * PL);
+    uint8_t *LF = lfl + (MASK ? buf * PL : 0);
     Tup Tc;
     uint8_t fc = CELL_FLUID | CELL_SOLVED;
-    double Uc[NF];
     if (valid) {
       double X[NF];
-      if (STAGE == 2) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) {
-          Uc[k] = U[k * fs + idx];
-          X[k] = Uc[k] + div_vol(g, dt * in.R[k * fs + idx]);  // main.cpp:421
-        }
-      } else {
-        in.load(g, idx, X);
-      }
+      pf.state(g, dt, X);
       make_tup(X, Tc);
       fc = cell_flags<MASK>(g, idx);
-      tup_store<PL>(lds, myslot, Tc);
-      if (MASK) lfl[myslot] = fc;
+      tup_store<PL>(L, myslot, Tc);
+      if (MASK) LF[myslot] = fc;
     }
     if (hslot >= 0) {
       double X[NF];
-      const int64_t hidx = hcol + m * sm;
-      in.load(g, hidx, X);
+      ph.state(g, dt, X);
       Tup Th;
       make_tup(X, Th);
-      tup_store<PL>(lds, hslot, Th);
-      if (MASK) lfl[hslot] = g.flags[hidx];
+      tup_store<PL>(L, hslot, Th);
+      if (MASK) LF[hslot] = g.flags[hcol + m * sm];
+    }
+
+    // ---- face between plane m-1 and plane m (evaluated once) ----
+    double Fz[NF];
+    bool zact = false;
+    if (valid) {
+      const bool hl = (m > m0) || have_prev0;
+      if (!hl) {  // lower border of the march axis
+        zact = fc & CELL_SOLVED;
+        if (zact) {
+          double lz = virtual_face(g, false, 2 * M, M, Tc, false, Fz, ctrl, true);
+          lam_max = (lz < lam_max) ? lam_max : lz;
+        }
+      } else {
+        zact = (fprev | fc) & CELL_SOLVED;
+        if (zact) {
+          double lz;
+          if (MASK && ((fprev ^ fc) & CELL_FLUID)) {
+            if (fc & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tc, false, Fz, ctrl, true);
+            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, Fz, ctrl, true);
+          } else {
+            lz = face_flux(Tp, Tc, M, Fz);
+            count_neg(Tp, Tc, ctrl);
+          }
+          lam_max = (lz < lam_max) ? lam_max : lz;
+        }
+      }
+
+      // ---- the previous cell is complete ----
+      if (m > m0) {
+        const int64_t pidx = idx - sm;
+        if (fprev & CELL_SOLVED) {
+#pragma unroll
+          for (int k = 0; k < NF; ++k)
+            out[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - A * Fz[k], dt);
+        } else if (STAGE != 3) {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) out[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+        }
+      }
+      // this cell's U, for its own update one plane later
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Uf[k] = pf.a[k];
+      } else if (STAGE == 3) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Uf[k] = U[k * fs + idx];
+      }
+    }
+
+    // next plane's inputs are in flight during this plane's faces
+    if (m + 1 < m1) {
+      if (valid) pf.issue(in, fs, idx + sm);
+      if (hslot >= 0) ph.issue(in, fs, hcol + (m + 1) * sm);
     }
     __syncthreads();
 
     if (valid) {
-      // face between plane m-1 and plane m (evaluated once, shared)
-      double Fz[NF];
-      const bool hl = (m > m0) || have_prev;
-      const bool zact = march_face<MASK>(g, M, hl, true, Tp, fprev, Tc, fc, Fz, lam_max, ctrl, count);
-
-      // the previous cell is complete
-      if (m > m0) {
-        const int64_t pidx = idx - sm;
-        if (fprev & CELL_SOLVED) {
-          const double A = g.A;
-#pragma unroll
-          for (int k = 0; k < NF; ++k) {
-            double rhs = (pxy[k] + zlo[k]) - A * Fz[k];
-            double u = (STAGE == 1) ? 0.0 : (STAGE == 2) ? Up[k] : U[k * fs + pidx];
-            out[k * fs + pidx] = rk_update<STAGE>(g, u, Tp.c[k], rhs, dt);
-          }
-        } else if (STAGE == 1) {
-#pragma unroll
-          for (int k = 0; k < NF; ++k) out[k * fs + pidx] = 0.0;
-        } else if (STAGE == 2) {
-#pragma unroll
-          for (int k = 0; k < NF; ++k) out[k * fs + pidx] = Up[k];
-        }
-      }
-
-      // in-plane faces of this cell
+      // ---- in-plane faces of this cell, reference order ----
       if (fc & CELL_SOLVED) {
         double acc[NF] = {0., 0., 0., 0., 0.};  // euler.cpp:142-148
-        inplane_faces<PL, MASK>(g, lds, lfl, 0, has_xm, has_xp, myslot - 1, myslot + 1, wrap_x, Tc, fc, acc,
-                                lam_max, ctrl, count);
-        if (DIM == 3)
-          inplane_faces<PL, MASK>(g, lds, lfl, 1, has_ym, has_yp, myslot - RW, myslot + RW, wrap_y, Tc, fc, acc,
-                                  lam_max, ctrl, count);
-        const double A = g.A;
 #pragma unroll
-        for (int k = 0; k < NF; ++k) {
-          pxy[k] = acc[k];
-          zlo[k] = zact ? A * Fz[k] : 0.0;
+        for (int a = 0; a < DIM - 1; ++a) {
+          const bool hm = (a == 0) ? has_xm : has_ym;
+          const bool hp = (a == 0) ? has_xp : has_yp;
+          const int sm_ = (a == 0) ? myslot - 1 : myslot - RW;
+          const int sp_ = (a == 0) ? myslot + 1 : myslot + RW;
+          const bool wrap = (a == 0) ? wrap_x : wrap_y;
+          double Fm[NF], Fp[NF];
+          {
+            const uint8_t fm = MASK && hm ? LF[sm_] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
+            const bool wall = MASK && hm && ((fm ^ fc) & CELL_FLUID);
+            double l;
+            if (hm && !wall) {
+              Tup Nb;
+              tup_load<PL>(L, sm_, Nb);
+              l = face_flux(Nb, Tc, a, Fm);
+              if (!(fm & CELL_SOLVED)) count_neg(Nb, Tc, ctrl);  // else the neighbour counts it
+            } else {
+              l = virtual_face(g, wall, 2 * a, a, Tc, false, Fm, ctrl, true);
+            }
+            lam_max = (l < lam_max) ? lam_max : l;
+          }
+          {
+            const uint8_t fp = MASK && hp ? LF[sp_] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
+            const bool wall = MASK && hp && ((fp ^ fc) & CELL_FLUID);
+            double l;
+            if (hp && !wall) {
+              Tup Nb;
+              tup_load<PL>(L, sp_, Nb);
+              l = face_flux(Tc, Nb, a, Fp);
+              count_neg(Tc, Nb, ctrl);
+            } else {
+              l = virtual_face(g, wall, 2 * a + 1, a, Tc, true, Fp, ctrl, true);
+            }
+            lam_max = (l < lam_max) ? lam_max : l;
+          }
+          if (wrap) {
+#pragma unroll
+            for (int k = 0; k < NF; ++k) { acc[k] -= A * Fp[k]; acc[k] += A * Fm[k]; }
+          } else {
+#pragma unroll
+            for (int k = 0; k < NF; ++k) { acc[k] += A * Fm[k]; acc[k] -= A * Fp[k]; }
+          }
+        }
+        if (top_wrap_here && m == nm - 1) {
+          // periodic top cell: its upper face (the wrap face) precedes its
+          // lower face in the reference's face list; finish the cell here
+          double X0[NF], Fw[NF];
+          Tup T0;
+          in.load(g, col, X0);
+          make_tup(X0, T0);
+          const uint8_t f0 = cell_flags<MASK>(g, col);
+          double lw;
+          if (MASK && ((f0 ^ fc) & CELL_FLUID)) {
+            lw = virtual_face(g, true, 2 * M + 1, M, Tc, true, Fw, ctrl, false);
+          } else {
+            lw = face_flux(Tc, T0, M, Fw);
+          }
+          lam_max = (lw < lam_max) ? lam_max : lw;
+#pragma unroll
+          for (int k = 0; k < NF; ++k) {
+            double rhs = (acc[k] - A * Fw[k]) + (zact ? A * Fz[k] : 0.0);
+            out[k * fs + idx] = rk_update<STAGE>(g, Uf[k], Tc.c[k], rhs, dt);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) part[k] = acc[k] + (zact ? A * Fz[k] : 0.0);
         }
       }
       Tp = Tc;
       fprev = fc;
-      if (STAGE == 2) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Up[k] = Uc[k];
-      }
     }
-    __syncthreads();
+    if (NBUF == 1) __syncthreads();
+    buf ^= (NBUF - 1);
   }
 
-  // epilogue: the upper face of the chunk's last cell
+  // ---- epilogue: the upper face of the chunk's last cell ----
   if (valid) {
     const int64_t last = m1 - 1;
     const int64_t pidx = col + last * sm;
-    int64_t mnext = m1;
-    bool have_next = true;
-    if (mnext >= nm) {
-      if (g.periodic[M]) mnext -= nm;
-      else have_next = false;
-    }
-    Tup Tn;
-    uint8_t fn = CELL_FLUID | CELL_SOLVED;
-    if (have_next) {
-      double X[NF];
-      in.load(g, col + mnext * sm, X);
-      make_tup(X, Tn);
-      fn = cell_flags<MASK>(g, col + mnext * sm);
-    }
-    double Fz[NF];
-    // counted here only at a non-periodic border: otherwise the next chunk's block counts it
-    march_face<MASK>(g, M, true, have_next, Tp, fprev, Tn, fn, Fz, lam_max, ctrl, !have_next);
-    if (fprev & CELL_SOLVED) {
-      const double A = g.A;
-      const bool wrap_z = g.periodic[M] && last == nm - 1;
+    if (top_wrap_here) {
+      if (!(fprev & CELL_SOLVED) && STAGE != 3) {  // solved top cells were finished in the loop
 #pragma unroll
-      for (int k = 0; k < NF; ++k) {
-        double rhs = wrap_z ? (pxy[k] - A * Fz[k]) + zlo[k] : (pxy[k] + zlo[k]) - A * Fz[k];
-        double u = (STAGE == 1) ? 0.0 : (STAGE == 2) ? Up[k] : U[k * fs + pidx];
-        out[k * fs + pidx] = rk_update<STAGE>(g, u, Tp.c[k], rhs, dt);
+        for (int k = 0; k < NF; ++k) out[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
       }
-    } else if (STAGE == 1) {
+    } else {
+      const bool have_next = m1 < nm;
+      double Fz[NF];
+      bool act;
+      double lz = 0.0;
+      if (have_next) {
+        double X[NF];
+        Tup Tn;
+        in.load(g, col + m1 * sm, X);
+        make_tup(X, Tn);
+        const uint8_t fn = cell_flags<MASK>(g, col + m1 * sm);
+        act = (fprev | fn) & CELL_SOLVED;
+        // counted by the block of the next chunk
+        if (act) {
+          if (MASK && ((fprev ^ fn) & CELL_FLUID)) {
+            if (fn & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tn, false, Fz, ctrl, false);
+            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, Fz, ctrl, false);
+          } else {
+            lz = face_flux(Tp, Tn, M, Fz);
+          }
+        }
+      } else {  // upper border of the march axis
+        act = fprev & CELL_SOLVED;
+        if (act) lz = virtual_face(g, false, 2 * M + 1, M, Tp, true, Fz, ctrl, true);
+      }
+      if (act) lam_max = (lz < lam_max) ? lam_max : lz;
+      if (fprev & CELL_SOLVED) {
 #pragma unroll
-      for (int k = 0; k < NF; ++k) out[k * fs + pidx] = 0.0;
-    } else if (STAGE == 2) {
+        for (int k = 0; k < NF; ++k)
+          out[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - A * Fz[k], dt);
+      } else if (STAGE != 3) {
 #pragma unroll
-      for (int k = 0; k < NF; ++k) out[k * fs + pidx] = Up[k];
+        for (int k = 0; k < NF; ++k) out[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+      }
     }
   }
   block_max_to<NT>(&ctrl->eig[slot_eig], lam_max, tid);
@@ -387,7 +446,8 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
   dim3 grid((unsigned)(ntx * nty * nch));
   dim3 block(TX, TY);
-  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx, nty);
+  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx,
+                     nty);
   return hipGetLastError();
 }
 
@@ -405,7 +465,7 @@ static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int sta
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
   if (g.order != 1 || sh.chunk < 1) return false;
   if (g.dim == 2) return sh.ty == 1;
-  return sh.ty == 4 || sh.ty == 8 || sh.ty == 16;
+  return sh.ty == 4 || sh.ty == 8 || sh.ty == 12 || sh.ty == 16;
 }
 
 MarchShape default_march_shape(const GridDev &g) {
@@ -415,7 +475,7 @@ MarchShape default_march_shape(const GridDev &g) {
     sh.chunk = 16;
   } else {
     sh.ty = 8;
-    sh.chunk = 32;
+    sh.chunk = 64;
   }
   return sh;
 }
@@ -432,6 +492,9 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &sh, int stage,
     case 4:
       return mask ? launch_march_s<3, 4, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
                   : launch_march_s<3, 4, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 12:
+      return mask ? launch_march_s<3, 12, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                  : launch_march_s<3, 12, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
     case 16:
       return mask ? launch_march_s<3, 16, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
                   : launch_march_s<3, 16, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
