@@ -189,6 +189,8 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
     int e2 = 0;
     g.pow2V = std::frexp(g.V, &e2) == 0.5 ? 1 : 0;
     g.invV = 1.0 / g.V;
+    g.pow2A = std::frexp(g.A, &e2) == 0.5 ? 1 : 0;
+    g.halfA = 0.5 * g.A;
   }
   ctx->shape = mmf::default_march_shape(g);
 

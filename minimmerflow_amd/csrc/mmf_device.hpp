@@ -180,6 +180,38 @@ __device__ __forceinline__ double face_flux(const Tup &L, const Tup &R, int a, d
   return lam;
 }
 
+// The residual contribution A*F of a face (euler.cpp:239,245), F as in
+// face_flux.  With A a power of two, A*(0.5*x) is computed as (0.5*A)*x: the
+// same double (both are exact scalings), one multiply fewer per component.
+__device__ __forceinline__ double face_flux_area(const Tup &L, const Tup &R, int a, bool pow2A, double A,
+                                                 double halfA, double *AF) {
+  const double unL = L.vel[a], unR = R.vel[a];
+  const double lambdaL = fabs(unL) + L.a;
+  const double lambdaR = fabs(unR) + R.a;
+  const double lam = (lambdaR < lambdaL) ? lambdaL : lambdaR;
+  const double mfL = L.c[FID_RHO] * unL;
+  const double mfR = R.c[FID_RHO] * unR;
+  double X[NF];
+  X[0] = (mfR + mfL) - lam * (R.c[0] - L.c[0]);
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    double fLb = mfL * L.vel[b];
+    double fRb = mfR * R.vel[b];
+    if (b == a) {
+      fLb = fLb + L.p;
+      fRb = fRb + R.p;
+    }
+    X[1 + b] = (fRb + fLb) - lam * (R.c[1 + b] - L.c[1 + b]);
+  }
+  X[4] = (unR * R.H + unL * L.H) - lam * (R.c[4] - L.c[4]);
+#pragma unroll
+  for (int k = 0; k < NF; ++k) AF[k] = pow2A ? halfA * X[k] : A * (0.5 * X[k]);
+  return lam;
+}
+
+// true when the state has a negative pressure or density (euler.cpp:94-101)
+__device__ __forceinline__ bool tup_negative(const Tup &t) { return (t.p < 0.) | (t.c[FID_RHO] < 0.); }
+
 // Positive doubles order like their bit patterns: max-reduce through u64.
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 

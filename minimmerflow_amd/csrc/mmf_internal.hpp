@@ -23,6 +23,8 @@ struct GridDev {
   const uint8_t *flags;  // MMF_CELL_* bits per cell, or nullptr (all fluid+solved)
   double invV;         // 1/V, used only when V is a power of two (x/V == x*invV exactly)
   int32_t pow2V;
+  int32_t pow2A;       // A is a power of two: A*(0.5*x) == (0.5*A)*x exactly
+  double halfA;        // 0.5*A
 };
 
 // Stage-kernel variants.

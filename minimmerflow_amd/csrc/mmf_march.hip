@@ -11,12 +11,16 @@
 //   2. the face between this plane and the previous one uses the previous
 //      tuple kept in registers, so each march-axis face is evaluated once; it
 //      completes the previous cell, whose RK stage update is written;
-//   3. the x and y faces of each cell read the neighbour tuples from LDS.
+//   3. after one barrier, the x and y faces of each cell read the neighbour
+//      tuples from LDS.  Tuple planes are double-buffered in LDS (TY <= 8), so
+//      one barrier per plane suffices.
 // Residual contributions are accumulated per cell in the reference's face
 // order (x lower, x upper, y lower, y upper, z lower, z upper), so the result
 // is bit-identical to the gather kernel and to the oracle; no float atomics.
 // Blocks are numbered so that each of the 8 XCDs takes a contiguous range of
 // tiles (neighbouring tiles share that XCD's L2 for the halo reads).
+// Cell offsets are 32-bit on uniform per-field base pointers (grids below
+// 2^29 cells; larger grids use the gather kernel).
 #include <hip/hip_runtime.h>
 
 #include "mmf_device.hpp"
@@ -29,6 +33,7 @@ namespace {
 
 constexpr int TX = 64;
 constexpr int NTUP = 11;  // c[5], vel[3], p, a, H
+constexpr int64_t MARCH_MAX_CELLS = (int64_t)1 << 29;
 
 template <int PL>
 __device__ __forceinline__ void tup_store(double *lds, int slot, const Tup &t) {
@@ -54,11 +59,70 @@ __device__ __forceinline__ void tup_load(const double *lds, int slot, Tup &t) {
   t.H = lds[10 * PL + slot];
 }
 
-// Flux of a face whose other side is a virtual state (border side 'side' or
-// a wall): the virtual state is built from the fluid tuple F.  fluid_lower:
-// the fluid cell is the lower (L) side of the face.
+// One SoA state: a uniform base and field stride; the per-lane part of an
+// address is a 32-bit byte offset (scalar-base + vector-offset addressing).
+struct Fields {
+  const char *base;
+  uint64_t fsb;  // field stride in bytes
+  __device__ __forceinline__ void set(const double *b, int64_t fs) {
+    base = (const char *)b;
+    fsb = (uint64_t)fs * sizeof(double);
+  }
+  __device__ __forceinline__ double at(int k, uint32_t i) const {
+    return *(const double *)(base + k * fsb + (uint32_t)(i * (uint32_t)sizeof(double)));
+  }
+};
+
+struct OutFields {
+  char *base;
+  uint64_t fsb;
+  __device__ __forceinline__ void set(double *b, int64_t fs) {
+    base = (char *)b;
+    fsb = (uint64_t)fs * sizeof(double);
+  }
+  __device__ __forceinline__ void put(int k, uint32_t i, double v) const {
+    *(double *)(base + k * fsb + (uint32_t)(i * (uint32_t)sizeof(double))) = v;
+  }
+};
+
+// Stage input of one cell from the stage's source fields.
+template <int STAGE>
+struct Source {
+  Fields X;  // U (stage 1) or W (stage 3)
+  Fields Uf, Rf;  // stage 2: W1 = U + dt R / V
+  __device__ __forceinline__ void load(const GridDev &g, double dt, uint32_t i, double *c) const {
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      c[k] = (STAGE == 2) ? Uf.at(k, i) + div_vol(g, dt * Rf.at(k, i)) : X.at(k, i);  // main.cpp:421
+  }
+};
+
+template <int STAGE>
+struct Prefetch {
+  double a[NF];  // X (stages 1, 3) or U (stage 2)
+  double b[NF];  // R (stage 2)
+  __device__ __forceinline__ void issue(const Source<STAGE> &src, uint32_t i) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      if (STAGE == 2) {
+        a[k] = src.Uf.at(k, i);
+        b[k] = src.Rf.at(k, i);
+      } else {
+        a[k] = src.X.at(k, i);
+      }
+    }
+  }
+  __device__ __forceinline__ void state(const GridDev &g, double dt, double *X) const {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) X[k] = (STAGE == 2) ? a[k] + div_vol(g, dt * b[k]) : a[k];
+  }
+};
+
+// Area-weighted flux AF of a face whose other side is a virtual state (border
+// side 'side' or a wall), built from the fluid tuple Fl.  fluid_lower: the
+// fluid cell is the lower (L) side of the face.
 __device__ __forceinline__ double virtual_face(const GridDev &g, bool wall, int side, int a, const Tup &Fl,
-                                            bool fluid_lower, double *F, Ctrl *ctrl, bool count) {
+                                               bool fluid_lower, double *AF, Ctrl *ctrl, bool count) {
   Tup V;
   if (!wall && g.bc[side] == BC_FREE_FLOW) {  // euler.cpp:298-308: a copy
     V = Fl;
@@ -70,37 +134,14 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, bool wall, int 
   }
   double lam;
   if (fluid_lower) {
-    lam = face_flux(Fl, V, a, F);
+    lam = face_flux_area(Fl, V, a, g.pow2A, g.A, g.halfA, AF);
     if (count) count_neg(Fl, V, ctrl);
   } else {
-    lam = face_flux(V, Fl, a, F);
+    lam = face_flux_area(V, Fl, a, g.pow2A, g.A, g.halfA, AF);
     if (count) count_neg(V, Fl, ctrl);
   }
   return lam;
 }
-
-template <int STAGE>
-struct Prefetch {
-  double a[NF];  // X (stages 1, 3) or U (stage 2)
-  double b[NF];  // R (stage 2)
-  __device__ __forceinline__ void issue(const StageIn<STAGE> &in, int64_t fs, int64_t idx) {
-    if (STAGE == 2) {
-#pragma unroll
-      for (int k = 0; k < NF; ++k) {
-        a[k] = in.U[k * fs + idx];
-        b[k] = in.R[k * fs + idx];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < NF; ++k) a[k] = in.X[k * fs + idx];
-    }
-  }
-  // the stage input state of the cell (main.cpp:421 for stage 2)
-  __device__ __forceinline__ void state(const GridDev &g, double dt, double *X) const {
-#pragma unroll
-    for (int k = 0; k < NF; ++k) X[k] = (STAGE == 2) ? a[k] + div_vol(g, dt * b[k]) : a[k];
-  }
-};
 
 }  // namespace
 
@@ -119,7 +160,15 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
   const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
-  in.dt = dt;  // the chunk-boundary loads (in.load) rebuild W1 with it
+
+  Source<STAGE> src;
+  src.X.set(in.X, g.fs);
+  src.Uf.set(in.U, g.fs);
+  src.Rf.set(in.R, g.fs);
+  Fields Uo;  // U for the RK update
+  Uo.set(U, g.fs);
+  OutFields outf;
+  outf.set(out, g.fs);
 
   // XCD-aware tile numbering
   const int nblk = gridDim.x;
@@ -131,30 +180,29 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 
   const int lane = threadIdx.x, row = threadIdx.y;
   const int tid = row * TX + lane;
-  const int64_t nx = g.n[0];
-  const int64_t ny = (DIM == 3) ? g.n[1] : 1;
-  const int64_t nm = g.n[M];
-  const int64_t fs = g.fs;
-  const int64_t x0 = (int64_t)bx * TX;
-  const int64_t y0 = (int64_t)by * TY;
-  const int64_t m0 = (int64_t)bz * chunk;
-  const int64_t m1 = (m0 + chunk < nm) ? m0 + chunk : nm;
-  const int w = (int)((nx - x0 < TX) ? nx - x0 : TX);
-  const int hgt = (DIM == 3) ? (int)((ny - y0 < TY) ? ny - y0 : TY) : 1;
+  const int nx = (int)g.n[0];
+  const int ny = (DIM == 3) ? (int)g.n[1] : 1;
+  const int nm = (int)g.n[M];
+  const int x0 = bx * TX;
+  const int y0 = by * TY;
+  const int m0 = bz * chunk;
+  const int m1 = (m0 + chunk < nm) ? m0 + chunk : nm;
+  const int w = (nx - x0 < TX) ? nx - x0 : TX;
+  const int hgt = (DIM == 3) ? ((ny - y0 < TY) ? ny - y0 : TY) : 1;
   const bool valid = lane < w && row < hgt;
-  const int64_t gx = x0 + lane;
-  const int64_t gy = (DIM == 3) ? y0 + row : 0;
-  const int64_t sm = (DIM == 3) ? nx * ny : nx;  // stride of the march axis
-  const int64_t col = gx + nx * gy;
+  const int gx = x0 + lane;
+  const int gy = (DIM == 3) ? y0 + row : 0;
+  const uint32_t sm = (uint32_t)nx * (uint32_t)ny;  // stride of the march axis
+  const uint32_t col = (uint32_t)gx + (uint32_t)nx * (uint32_t)gy;
   const int myslot = (row + 1) * RW + lane + 1;
 
   // halo ring cell handled by this thread (one per plane), if any
-  int64_t hcol = 0;
+  uint32_t hcol = 0;
   int hslot = -1;
   {
     const int H = (DIM == 3) ? 2 * w + 2 * hgt : 2;
     if (tid < H) {
-      int64_t hx, hy = 0;
+      int hx, hy = 0;
       int sr = 1, sc;
       if (DIM == 3) {
         if (tid < w) { hx = x0 + tid; hy = y0 - 1; sr = 0; sc = tid + 1; }
@@ -175,7 +223,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         else ok = false;
       }
       if (ok) {
-        hcol = hx + nx * hy;
+        hcol = (uint32_t)hx + (uint32_t)nx * (uint32_t)hy;
         hslot = sr * RW + sc;
       }
     }
@@ -188,18 +236,20 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const bool has_yp = DIM == 3 && valid && (gy + 1 < ny || g.periodic[1]);
   const bool wrap_y = DIM == 3 && g.periodic[1] && gy == ny - 1;
   const bool top_wrap_here = g.periodic[M] && m1 == nm;  // this chunk holds the periodic top cell
-  const double A = g.A;
+  const bool pow2A = g.pow2A;
+  const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
-  Tup Tp;                                    // previous plane's tuple (this column)
+  Tup Tp;  // previous plane's tuple (this column)
   uint8_t fprev = CELL_FLUID | CELL_SOLVED;
-  double part[NF];                           // residual of the previous cell so far
+  bool bad_prev = false;  // previous tuple has a negative p or rho
+  double part[NF];        // residual of the previous cell so far
 #pragma unroll
   for (int k = 0; k < NF; ++k) part[k] = 0.0;
 
   // prologue: the plane below the chunk
   {
-    int64_t mprev = m0 - 1;
+    int mprev = m0 - 1;
     bool have_prev = true;
     if (mprev < 0) {
       if (g.periodic[M]) mprev += nm;
@@ -207,55 +257,72 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     }
     if (valid && have_prev) {
       double X[NF];
-      in.load(g, col + mprev * sm, X);
+      src.load(g, dt, col + (uint32_t)mprev * sm, X);
       make_tup(X, Tp);
-      fprev = cell_flags<MASK>(g, col + mprev * sm);
+      fprev = cell_flags<MASK>(g, col + (uint32_t)mprev * sm);
+      bad_prev = tup_negative(Tp);
     }
   }
   const bool have_prev0 = (m0 > 0) || g.periodic[M];
 
   Prefetch<STAGE> pf, ph;
-  if (valid) pf.issue(in, fs, col + m0 * sm);
-  if (hslot >= 0) ph.issue(in, fs, hcol + m0 * sm);
+  if (valid) pf.issue(src, col + (uint32_t)m0 * sm);
+  if (hslot >= 0) ph.issue(src, hcol + (uint32_t)m0 * sm);
 
-  // U of the previous cell for its RK update (stages 2 and 3), read one plane ahead
-  double Uf[NF];
+  // U of the previous / current cell for the RK update (stages 2 and 3)
+  double Uf[NF], Ucur[NF];
 #pragma unroll
-  for (int k = 0; k < NF; ++k) Uf[k] = 0.0;
+  for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k] = 0.0;
 
   int buf = 0;
-  for (int64_t m = m0; m < m1; ++m) {
-    const int64_t idx = col + m * sm;
+  for (int m = m0; m < m1; ++m) {
+    const uint32_t idx = col + (uint32_t)m * sm;
     double *L = lds + buf * (NTUP * PL);
     uint8_t *LF = lfl + (MASK ? buf * PL : 0);
     Tup Tc;
     uint8_t fc = CELL_FLUID | CELL_SOLVED;
+    bool bad_c = false;
+    // take this plane's inputs and put the next plane's loads in flight at
+    // once: they then have a whole plane of work to arrive
+    double X[NF], Xh[NF];
     if (valid) {
-      double X[NF];
       pf.state(g, dt, X);
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = pf.a[k];
+      } else if (STAGE == 3) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
+      }
+    }
+    if (hslot >= 0) ph.state(g, dt, Xh);
+    if (m + 1 < m1) {
+      if (valid) pf.issue(src, idx + sm);
+      if (hslot >= 0) ph.issue(src, hcol + (uint32_t)(m + 1) * sm);
+    }
+    if (valid) {
       make_tup(X, Tc);
+      bad_c = tup_negative(Tc);
       fc = cell_flags<MASK>(g, idx);
       tup_store<PL>(L, myslot, Tc);
       if (MASK) LF[myslot] = fc;
     }
     if (hslot >= 0) {
-      double X[NF];
-      ph.state(g, dt, X);
       Tup Th;
-      make_tup(X, Th);
+      make_tup(Xh, Th);
       tup_store<PL>(L, hslot, Th);
-      if (MASK) LF[hslot] = g.flags[hcol + m * sm];
+      if (MASK) LF[hslot] = g.flags[hcol + (uint32_t)m * sm];
     }
 
     // ---- face between plane m-1 and plane m (evaluated once) ----
-    double Fz[NF];
+    double AFz[NF];
     bool zact = false;
     if (valid) {
       const bool hl = (m > m0) || have_prev0;
       if (!hl) {  // lower border of the march axis
         zact = fc & CELL_SOLVED;
         if (zact) {
-          double lz = virtual_face(g, false, 2 * M, M, Tc, false, Fz, ctrl, true);
+          double lz = virtual_face(g, false, 2 * M, M, Tc, false, AFz, ctrl, true);
           lam_max = (lz < lam_max) ? lam_max : lz;
         }
       } else {
@@ -263,11 +330,11 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         if (zact) {
           double lz;
           if (MASK && ((fprev ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tc, false, Fz, ctrl, true);
-            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, Fz, ctrl, true);
+            if (fc & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tc, false, AFz, ctrl, true);
+            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
           } else {
-            lz = face_flux(Tp, Tc, M, Fz);
-            count_neg(Tp, Tc, ctrl);
+            lz = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFz);
+            if (bad_prev | bad_c) count_neg(Tp, Tc, ctrl);
           }
           lam_max = (lz < lam_max) ? lam_max : lz;
         }
@@ -275,30 +342,20 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 
       // ---- the previous cell is complete ----
       if (m > m0) {
-        const int64_t pidx = idx - sm;
+        const uint32_t pidx = idx - sm;
         if (fprev & CELL_SOLVED) {
 #pragma unroll
-          for (int k = 0; k < NF; ++k)
-            out[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - A * Fz[k], dt);
+          for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
         } else if (STAGE != 3) {
 #pragma unroll
-          for (int k = 0; k < NF; ++k) out[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+          for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
         }
       }
       // this cell's U, for its own update one plane later
-      if (STAGE == 2) {
+      if (STAGE >= 2) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) Uf[k] = pf.a[k];
-      } else if (STAGE == 3) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Uf[k] = U[k * fs + idx];
+        for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k];
       }
-    }
-
-    // next plane's inputs are in flight during this plane's faces
-    if (m + 1 < m1) {
-      if (valid) pf.issue(in, fs, idx + sm);
-      if (hslot >= 0) ph.issue(in, fs, hcol + (m + 1) * sm);
     }
     __syncthreads();
 
@@ -313,7 +370,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           const int sm_ = (a == 0) ? myslot - 1 : myslot - RW;
           const int sp_ = (a == 0) ? myslot + 1 : myslot + RW;
           const bool wrap = (a == 0) ? wrap_x : wrap_y;
-          double Fm[NF], Fp[NF];
+          double AFm[NF], AFp[NF];
           {
             const uint8_t fm = MASK && hm ? LF[sm_] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
             const bool wall = MASK && hm && ((fm ^ fc) & CELL_FLUID);
@@ -321,10 +378,11 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
             if (hm && !wall) {
               Tup Nb;
               tup_load<PL>(L, sm_, Nb);
-              l = face_flux(Nb, Tc, a, Fm);
-              if (!(fm & CELL_SOLVED)) count_neg(Nb, Tc, ctrl);  // else the neighbour counts it
+              l = face_flux_area(Nb, Tc, a, pow2A, A, halfA, AFm);
+              // counted here only when the neighbour does not evaluate this face
+              if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
             } else {
-              l = virtual_face(g, wall, 2 * a, a, Tc, false, Fm, ctrl, true);
+              l = virtual_face(g, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -335,48 +393,49 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
             if (hp && !wall) {
               Tup Nb;
               tup_load<PL>(L, sp_, Nb);
-              l = face_flux(Tc, Nb, a, Fp);
-              count_neg(Tc, Nb, ctrl);
+              l = face_flux_area(Tc, Nb, a, pow2A, A, halfA, AFp);
+              if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
             } else {
-              l = virtual_face(g, wall, 2 * a + 1, a, Tc, true, Fp, ctrl, true);
+              l = virtual_face(g, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
           if (wrap) {
 #pragma unroll
-            for (int k = 0; k < NF; ++k) { acc[k] -= A * Fp[k]; acc[k] += A * Fm[k]; }
+            for (int k = 0; k < NF; ++k) { acc[k] -= AFp[k]; acc[k] += AFm[k]; }
           } else {
 #pragma unroll
-            for (int k = 0; k < NF; ++k) { acc[k] += A * Fm[k]; acc[k] -= A * Fp[k]; }
+            for (int k = 0; k < NF; ++k) { acc[k] += AFm[k]; acc[k] -= AFp[k]; }
           }
         }
         if (top_wrap_here && m == nm - 1) {
           // periodic top cell: its upper face (the wrap face) precedes its
           // lower face in the reference's face list; finish the cell here
-          double X0[NF], Fw[NF];
+          double X0[NF], AFw[NF];
           Tup T0;
-          in.load(g, col, X0);
+          src.load(g, dt, col, X0);
           make_tup(X0, T0);
           const uint8_t f0 = cell_flags<MASK>(g, col);
           double lw;
           if (MASK && ((f0 ^ fc) & CELL_FLUID)) {
-            lw = virtual_face(g, true, 2 * M + 1, M, Tc, true, Fw, ctrl, false);
+            lw = virtual_face(g, true, 2 * M + 1, M, Tc, true, AFw, ctrl, false);
           } else {
-            lw = face_flux(Tc, T0, M, Fw);
+            lw = face_flux_area(Tc, T0, M, pow2A, A, halfA, AFw);
           }
           lam_max = (lw < lam_max) ? lam_max : lw;
 #pragma unroll
           for (int k = 0; k < NF; ++k) {
-            double rhs = (acc[k] - A * Fw[k]) + (zact ? A * Fz[k] : 0.0);
-            out[k * fs + idx] = rk_update<STAGE>(g, Uf[k], Tc.c[k], rhs, dt);
+            double rhs = (acc[k] - AFw[k]) + (zact ? AFz[k] : 0.0);
+            outf.put(k, idx, rk_update<STAGE>(g, Uf[k], Tc.c[k], rhs, dt));
           }
         } else {
 #pragma unroll
-          for (int k = 0; k < NF; ++k) part[k] = acc[k] + (zact ? A * Fz[k] : 0.0);
+          for (int k = 0; k < NF; ++k) part[k] = acc[k] + (zact ? AFz[k] : 0.0);
         }
       }
       Tp = Tc;
       fprev = fc;
+      bad_prev = bad_c;
     }
     if (NBUF == 1) __syncthreads();
     buf ^= (NBUF - 1);
@@ -384,46 +443,45 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 
   // ---- epilogue: the upper face of the chunk's last cell ----
   if (valid) {
-    const int64_t last = m1 - 1;
-    const int64_t pidx = col + last * sm;
+    const uint32_t pidx = col + (uint32_t)(m1 - 1) * sm;
     if (top_wrap_here) {
       if (!(fprev & CELL_SOLVED) && STAGE != 3) {  // solved top cells were finished in the loop
 #pragma unroll
-        for (int k = 0; k < NF; ++k) out[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+        for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
       }
     } else {
       const bool have_next = m1 < nm;
-      double Fz[NF];
+      double AFz[NF];
       bool act;
       double lz = 0.0;
       if (have_next) {
         double X[NF];
         Tup Tn;
-        in.load(g, col + m1 * sm, X);
+        const uint32_t nidx = col + (uint32_t)m1 * sm;
+        src.load(g, dt, nidx, X);
         make_tup(X, Tn);
-        const uint8_t fn = cell_flags<MASK>(g, col + m1 * sm);
+        const uint8_t fn = cell_flags<MASK>(g, nidx);
         act = (fprev | fn) & CELL_SOLVED;
         // counted by the block of the next chunk
         if (act) {
           if (MASK && ((fprev ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tn, false, Fz, ctrl, false);
-            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, Fz, ctrl, false);
+            if (fn & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tn, false, AFz, ctrl, false);
+            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, AFz, ctrl, false);
           } else {
-            lz = face_flux(Tp, Tn, M, Fz);
+            lz = face_flux_area(Tp, Tn, M, pow2A, A, halfA, AFz);
           }
         }
       } else {  // upper border of the march axis
         act = fprev & CELL_SOLVED;
-        if (act) lz = virtual_face(g, false, 2 * M + 1, M, Tp, true, Fz, ctrl, true);
+        if (act) lz = virtual_face(g, false, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
       }
       if (act) lam_max = (lz < lam_max) ? lam_max : lz;
       if (fprev & CELL_SOLVED) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k)
-          out[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - A * Fz[k], dt);
+        for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
       } else if (STAGE != 3) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) out[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+        for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
       }
     }
   }
@@ -463,7 +521,7 @@ static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int sta
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
-  if (g.order != 1 || sh.chunk < 1) return false;
+  if (g.order != 1 || sh.chunk < 1 || g.N >= MARCH_MAX_CELLS) return false;
   if (g.dim == 2) return sh.ty == 1;
   return sh.ty == 4 || sh.ty == 8 || sh.ty == 12 || sh.ty == 16;
 }
@@ -475,7 +533,7 @@ MarchShape default_march_shape(const GridDev &g) {
     sh.chunk = 16;
   } else {
     sh.ty = 8;
-    sh.chunk = 64;
+    sh.chunk = 128;
   }
   return sh;
 }
