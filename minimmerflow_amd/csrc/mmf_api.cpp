@@ -248,7 +248,7 @@ int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk)
   if (tile_rows > 0) sh.ty = tile_rows;
   if (chunk > 0) sh.chunk = chunk;
   if (kind == mmf::KERNEL_MARCH && !mmf::march_shape_supported(ctx->g, sh))
-    return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1 only; tile rows 4/8/16 in 3D, 1 in 2D");
+    return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1 only, fewer than 2^29 cells; tile rows 4 or 8 in 3D, 1 in 2D");
   ctx->kernel = kind;
   ctx->shape = sh;
   return MMF_OK;

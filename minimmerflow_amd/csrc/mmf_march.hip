@@ -145,7 +145,7 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, bool wall, int 
 
 }  // namespace
 
-template <int DIM, int TY, int STAGE, bool MASK>
+template <int DIM, int TY, int STAGE, bool MASK, bool P2>
 __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
                                                   Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
   constexpr int NT = TX * TY;
@@ -236,7 +236,11 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const bool has_yp = DIM == 3 && valid && (gy + 1 < ny || g.periodic[1]);
   const bool wrap_y = DIM == 3 && g.periodic[1] && gy == ny - 1;
   const bool top_wrap_here = g.periodic[M] && m1 == nm;  // this chunk holds the periodic top cell
-  const bool pow2A = g.pow2A;
+  // P2: A and V are powers of two (compile-time, so no select between the forms)
+  const bool pow2A = P2;
+  GridDev gp = g;
+  gp.pow2A = P2;
+  gp.pow2V = P2;
   const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     }
     if (valid && have_prev) {
       double X[NF];
-      src.load(g, dt, col + (uint32_t)mprev * sm, X);
+      src.load(gp, dt, col + (uint32_t)mprev * sm, X);
       make_tup(X, Tp);
       fprev = cell_flags<MASK>(g, col + (uint32_t)mprev * sm);
       bad_prev = tup_negative(Tp);
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     // once: they then have a whole plane of work to arrive
     double X[NF], Xh[NF];
     if (valid) {
-      pf.state(g, dt, X);
+      pf.state(gp, dt, X);
       if (STAGE == 2) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) Ucur[k] = pf.a[k];
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
       }
     }
-    if (hslot >= 0) ph.state(g, dt, Xh);
+    if (hslot >= 0) ph.state(gp, dt, Xh);
     if (m + 1 < m1) {
       if (valid) pf.issue(src, idx + sm);
       if (hslot >= 0) ph.issue(src, hcol + (uint32_t)(m + 1) * sm);
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       if (!hl) {  // lower border of the march axis
         zact = fc & CELL_SOLVED;
         if (zact) {
-          double lz = virtual_face(g, false, 2 * M, M, Tc, false, AFz, ctrl, true);
+          double lz = virtual_face(gp, false, 2 * M, M, Tc, false, AFz, ctrl, true);
           lam_max = (lz < lam_max) ? lam_max : lz;
         }
       } else {
@@ -330,8 +334,8 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         if (zact) {
           double lz;
           if (MASK && ((fprev ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tc, false, AFz, ctrl, true);
-            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
+            if (fc & CELL_FLUID) lz = virtual_face(gp, true, 2 * M, M, Tc, false, AFz, ctrl, true);
+            else lz = virtual_face(gp, true, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
           } else {
             lz = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFz);
             if (bad_prev | bad_c) count_neg(Tp, Tc, ctrl);
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         const uint32_t pidx = idx - sm;
         if (fprev & CELL_SOLVED) {
 #pragma unroll
-          for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
+          for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(gp, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
         } else if (STAGE != 3) {
 #pragma unroll
           for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
@@ -382,7 +386,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               // counted here only when the neighbour does not evaluate this face
               if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
             } else {
-              l = virtual_face(g, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
+              l = virtual_face(gp, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               l = face_flux_area(Tc, Nb, a, pow2A, A, halfA, AFp);
               if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
             } else {
-              l = virtual_face(g, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
+              l = virtual_face(gp, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -413,12 +417,12 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           // lower face in the reference's face list; finish the cell here
           double X0[NF], AFw[NF];
           Tup T0;
-          src.load(g, dt, col, X0);
+          src.load(gp, dt, col, X0);
           make_tup(X0, T0);
           const uint8_t f0 = cell_flags<MASK>(g, col);
           double lw;
           if (MASK && ((f0 ^ fc) & CELL_FLUID)) {
-            lw = virtual_face(g, true, 2 * M + 1, M, Tc, true, AFw, ctrl, false);
+            lw = virtual_face(gp, true, 2 * M + 1, M, Tc, true, AFw, ctrl, false);
           } else {
             lw = face_flux_area(Tc, T0, M, pow2A, A, halfA, AFw);
           }
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 #pragma unroll
           for (int k = 0; k < NF; ++k) {
             double rhs = (acc[k] - AFw[k]) + (zact ? AFz[k] : 0.0);
-            outf.put(k, idx, rk_update<STAGE>(g, Uf[k], Tc.c[k], rhs, dt));
+            outf.put(k, idx, rk_update<STAGE>(gp, Uf[k], Tc.c[k], rhs, dt));
           }
         } else {
 #pragma unroll
@@ -458,27 +462,27 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         double X[NF];
         Tup Tn;
         const uint32_t nidx = col + (uint32_t)m1 * sm;
-        src.load(g, dt, nidx, X);
+        src.load(gp, dt, nidx, X);
         make_tup(X, Tn);
         const uint8_t fn = cell_flags<MASK>(g, nidx);
         act = (fprev | fn) & CELL_SOLVED;
         // counted by the block of the next chunk
         if (act) {
           if (MASK && ((fprev ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) lz = virtual_face(g, true, 2 * M, M, Tn, false, AFz, ctrl, false);
-            else lz = virtual_face(g, true, 2 * M + 1, M, Tp, true, AFz, ctrl, false);
+            if (fn & CELL_FLUID) lz = virtual_face(gp, true, 2 * M, M, Tn, false, AFz, ctrl, false);
+            else lz = virtual_face(gp, true, 2 * M + 1, M, Tp, true, AFz, ctrl, false);
           } else {
             lz = face_flux_area(Tp, Tn, M, pow2A, A, halfA, AFz);
           }
         }
       } else {  // upper border of the march axis
         act = fprev & CELL_SOLVED;
-        if (act) lz = virtual_face(g, false, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
+        if (act) lz = virtual_face(gp, false, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
       }
       if (act) lam_max = (lz < lam_max) ? lam_max : lz;
       if (fprev & CELL_SOLVED) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
+        for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(gp, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
       } else if (STAGE != 3) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
@@ -488,7 +492,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   block_max_to<NT>(&ctrl->eig[slot_eig], lam_max, tid);
 }
 
-template <int DIM, int TY, int STAGE, bool MASK>
+template <int DIM, int TY, int STAGE, bool MASK, bool P2>
 static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mode, const double *U, const double *R,
                                  const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
   StageIn<STAGE> in;
@@ -504,26 +508,34 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
   dim3 grid((unsigned)(ntx * nty * nch));
   dim3 block(TX, TY);
-  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx,
+  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK, P2>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx,
                      nty);
   return hipGetLastError();
+}
+
+template <int DIM, int TY, bool MASK, bool P2>
+static hipError_t launch_march_p(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
+                                 const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
+                                 hipStream_t s) {
+  switch (stage) {
+    case 1: return launch_march_t<DIM, TY, 1, MASK, P2>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return launch_march_t<DIM, TY, 2, MASK, P2>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return launch_march_t<DIM, TY, 3, MASK, P2>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
 }
 
 template <int DIM, int TY, bool MASK>
 static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
-  switch (stage) {
-    case 1: return launch_march_t<DIM, TY, 1, MASK>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    case 2: return launch_march_t<DIM, TY, 2, MASK>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    default: return launch_march_t<DIM, TY, 3, MASK>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-  }
+  if (g.pow2A && g.pow2V) return launch_march_p<DIM, TY, MASK, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  return launch_march_p<DIM, TY, MASK, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
   if (g.order != 1 || sh.chunk < 1 || g.N >= MARCH_MAX_CELLS) return false;
   if (g.dim == 2) return sh.ty == 1;
-  return sh.ty == 4 || sh.ty == 8 || sh.ty == 12 || sh.ty == 16;
+  return sh.ty == 4 || sh.ty == 8;
 }
 
 MarchShape default_march_shape(const GridDev &g) {
@@ -550,12 +562,6 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &sh, int stage,
     case 4:
       return mask ? launch_march_s<3, 4, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
                   : launch_march_s<3, 4, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-    case 12:
-      return mask ? launch_march_s<3, 12, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
-                  : launch_march_s<3, 12, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-    case 16:
-      return mask ? launch_march_s<3, 16, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
-                  : launch_march_s<3, 16, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
     default:
       return mask ? launch_march_s<3, 8, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
                   : launch_march_s<3, 8, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);

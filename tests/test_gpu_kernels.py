@@ -11,7 +11,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(8, 32), (4, 7), (16, 64), (8, 1)]
+SHAPES = [(8, 32), (4, 7), (8, 128), (8, 1)]
 
 
 def build(mmf, dim, n, side_bc, info=None, body=None, order=1, seed=0):
