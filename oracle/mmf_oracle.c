@@ -130,9 +130,15 @@ static void orc_reflecting(const double *normal, const double *cons, double *con
     double un = orc_normal_velocity(prim, normal);
     double u_n[3] = {un * normal[0], un * normal[1], un * normal[2]};
 
+#ifdef ORC_PLANT_REFLECT /* planted mistake (tests only): tangential instead of normal mirror */
+    prim[ORC_U] = u[0] - 2 * (u[0] - u_n[0]);
+    prim[ORC_V] = u[1] - 2 * (u[1] - u_n[1]);
+    prim[ORC_W] = u[2] - 2 * (u[2] - u_n[2]);
+#else
     prim[ORC_U] = u[0] - 2 * u_n[0];
     prim[ORC_V] = u[1] - 2 * u_n[1];
     prim[ORC_W] = u[2] - 2 * u_n[2];
+#endif
 
     orc_prim2cons(prim, cons_bc);
 }
@@ -708,7 +714,11 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
     orc_compute_rhs(s, U, RHS, &maxEig);
     if (stage_eig) stage_eig[0] = maxEig;
 
+#ifdef ORC_PLANT_DT /* planted mistake (tests only): the 0.9 safety factor dropped */
+    double dt = cfl * minCellSize / maxEig;
+#else
     double dt = 0.9 * cfl * minCellSize / maxEig; /* main.cpp:398 */
+#endif
     if (t + dt > tmax) dt = tmax - t;             /* main.cpp:400-402 */
 
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:409-423 */
@@ -724,7 +734,11 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:445-459 */
         if (!s->solved[c]) continue;
         for (int k = 0; k < ORC_NF; ++k)
+#ifdef ORC_PLANT_RK /* planted mistake (tests only): stage-2 weights of a different scheme */
+            W[ORC_NF * c + k] = 0.5 * U[ORC_NF * c + k] + 0.5 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
+#else
             W[ORC_NF * c + k] = 0.75 * U[ORC_NF * c + k] + 0.25 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
+#endif
     }
 
     orc_compute_polynomials(s, W);

@@ -41,13 +41,46 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def build_variant(defines, outdir):
+    """Build a copy of the oracle with extra -D flags (used to plant a mistake)."""
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "liboracle_%s.so" % "_".join(d.lower() for d in defines))
+    cmd = ["gcc", "-O2", "-std=gnu99", "-fPIC", "-ffp-contract=off", "-shared"]
+    cmd += ["-D" + d for d in defines]
+    cmd += ["-o", out, os.path.join(HERE, "mmf_oracle.c"), "-lm"]
+    subprocess.run(cmd, check=True)
+    return out
+
+
+class use_library:
+    """Context manager: route this module's calls to another build of the oracle."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = None
+        _load(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+
+
 def lib():
-    global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    L = C.CDLL(LIB_PATH)
+    return _load(LIB_PATH)
+
+
+def _load(path):
+    global _lib
+    L = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     i64p = C.POINTER(C.c_int64)
     L.orc_problem_defaults.argtypes = [C.POINTER(Problem), C.c_int, C.c_int]
