@@ -48,7 +48,27 @@ def build(verbose=False):
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER = os.path.join(LIBDIR, "minimmerflow")
+
+
+def build_driver(verbose=False):
+    """The settings.xml front-end (host C++ only), linked against libmmf.so."""
+    src = os.path.join(CSRC, "driver", "main.cpp")
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", os.path.join(CSRC, "driver"), src,
+           "-L", LIBDIR, "-lmmf", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the minimmerflow driver")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
