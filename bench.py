@@ -1,19 +1,23 @@
 #!/usr/bin/env python3
 """Throughput of the finite-volume time step on one MI355X.
 
-Workload: 3D radial Sod, 256^3 uniform cells on [-1,1]^3, gamma 1.4, inside
-r<0.4 rho=1 p=1, outside rho=0.125 p=0.1, at rest, transmissive (free-flow)
-borders, first-order Rusanov + SSP-RK3 at CFL 0.5 towards t=0.25, FP64.
+Default workload: 3D radial Sod, 256^3 uniform cells on [-1,1]^3, gamma 1.4,
+inside r<0.4 rho=1 p=1, outside rho=0.125 p=0.1, at rest, transmissive
+(free-flow) borders, first-order Rusanov + SSP-RK3 at CFL 0.5 towards t=0.25,
+FP64.  --workload sphere192 runs the immersed-sphere case instead (192^3 on
+[-2,2]^3, sphere r=0.25, Mach 0.3 free stream, Dirichlet inflow at x-,
+free-flow elsewhere).
 
 A step is one SSP-RK3 time step (three fused flux+update stage kernels and
 the device-side dt/t update) over the whole grid, with the state resident
-in HBM.  value = cells x 3 stages x steps / wall time of the timed steps
-(cell-updates per second per RK stage).
+in HBM.  value = solved cells x 3 stages x steps / wall time of the timed
+steps (cell-updates per second per RK stage).
 
 Prints ONE JSON line on stdout.  The roofline entry is for the stage kernel:
 achieved = algorithmic bytes per launch (80/120/120 B per cell for stages
-1/2/3, mean 106.67 B) / average launch duration from HIP events recorded on
-the solver's stream around every stage launch of the timed steps.
+1/2/3, mean 106.67 B; +1 B per cell for the body mask) / average launch
+duration from HIP events recorded on the solver's stream around every stage
+launch of the timed steps.
 """
 import argparse
 import json
@@ -27,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_STAGE = (80.0 + 120.0 + 120.0) / 3.0
+GAMMA = 1.4
 
 
 def log(msg):
@@ -44,8 +49,37 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+# ---- workloads ----------------------------------------------------------------
+
+def radsod_setup(mmf, n, order):
+    pb = mmf.problem("radsod", 3, sod_r0=0.4, domain_origin=[-1.0, -1.0, -1.0], domain_length=2.0)
+    s = mmf.Solver.for_problem(pb, n, order=order, side_bc=[mmf.BC_FREE_FLOW] * 6)
+    s.set_initial_state(pb)
+    s.set_time(0.0, 0.25, 0.5)
+    cfg = {"domain": "[-1,1]^3", "initial": "r<0.4: rho=1 p=1; else rho=0.125 p=0.1; u=0",
+           "boundaries": "transmissive (free flow)", "time": "SSP-RK3, CFL 0.5, t_end 0.25"}
+    return s, pb, 0.0, cfg
+
+
+def sphere_setup(mmf, n, order):
+    fs = [1.0 / GAMMA, 0.3, 0.0, 0.0, 1.0 / GAMMA]  # rho = 1, p = 1/gamma, a = 1, Mach 0.3
+    pb = mmf.problem("freestream", 3, fs_prim=fs, domain_origin=[-2.0, -2.0, -2.0], domain_length=4.0)
+    side_bc = [mmf.BC_DIRICHLET] + [mmf.BC_FREE_FLOW] * 5
+    s = mmf.Solver(3, [n] * 3, [-2.0, -2.0, -2.0], 4.0 / n, side_bc, [fs] * 6, order)
+    s.set_bodies(spheres=[[0.0, 0.0, 0.0, 0.25]])
+    s.set_initial_state(pb)
+    s.set_time(0.0, 1e30, 0.5)  # the case runs a fixed number of steps
+    cfg = {"domain": "[-2,2]^3", "body": "sphere r=0.25 at the origin (level set at centroids)",
+           "initial": "free stream rho=1, p=1/gamma, u=0.3 (Mach 0.3)",
+           "boundaries": "Dirichlet inflow at x-, free flow elsewhere", "time": "SSP-RK3, CFL 0.5, 2000 steps"}
+    return s, pb, 1.0, cfg
+
+
+WORKLOADS = {"radsod256": (radsod_setup, 256, "radial_sod_3d"), "sphere192": (sphere_setup, 192, "sphere_3d")}
+
+
 def cpu_baseline(n, steps):
-    """The oracle (plain C, one thread) on a bounded sample of the same workload."""
+    """The oracle (plain C, one thread) on a bounded sample of the radial-Sod workload."""
     from oracle import oracle as O
     pb = O.problem(O.RADSOD, 3, sod_r0=0.4)
     s = O.Solver(pb, 1, [n] * 3, [-1.0, -1.0, -1.0], 2.0 / n, side_bc=[O.BC_FREE_FLOW] * 6)
@@ -83,23 +117,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="radsod256")
+    ap.add_argument("--n", type=int, default=0, help="cells per axis (default: the workload's)")
     ap.add_argument("--order", type=int, default=1, choices=(1, 2))
     ap.add_argument("--kernel", choices=("auto", "gather", "march"), default="auto")
-    ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4/8/16)")
+    ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4 or 8)")
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()
-
-    if args.copy_calibration:
-        import minimmerflow_amd as mmf
-        ms = mmf.calibration_copy(args.copy_calibration, reps=5)
-        print(json.dumps({"copy_bytes": args.copy_calibration, "ms_per_copy": ms,
-                          "GBs_read_plus_write": 2 * args.copy_calibration / (ms * 1e-3) / 1e9}), flush=True)
-        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
@@ -108,17 +136,23 @@ def main():
 
     import minimmerflow_amd as mmf
 
-    n = args.n
+    if args.copy_calibration:
+        ms = mmf.calibration_copy(args.copy_calibration, reps=5)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "ms_per_copy": ms,
+                          "GBs_read_plus_write": 2 * args.copy_calibration / (ms * 1e-3) / 1e9}), flush=True)
+        return
+
+    setup, n_default, wname = WORKLOADS[args.workload]
+    n = args.n or n_default
     N = n ** 3
-    pb = mmf.problem("radsod", 3, sod_r0=0.4, domain_origin=[-1.0, -1.0, -1.0], domain_length=2.0)
-    s = mmf.Solver.for_problem(pb, n, order=args.order, side_bc=[mmf.BC_FREE_FLOW] * 6)
+    t_setup = time.perf_counter()
+    s, pb, mask_bytes, cfg = setup(mmf, n, args.order)
     kind = {"auto": mmf.KERNEL_AUTO, "gather": mmf.KERNEL_GATHER, "march": mmf.KERNEL_MARCH}[args.kernel]
     s.set_kernel(kind, args.rows, args.chunk)
     kinfo = s.kernel()
-    t_setup = time.perf_counter()
-    s.set_initial_state(pb)
-    s.set_time(0.0, 0.25, 0.5)
-    log("setup %.1fs: %d^3 cells, %d interfaces, order %d" % (time.perf_counter() - t_setup, n, s.ninterfaces, args.order))
+    solved = N if mask_bytes == 0 else int(((s.cell_flags() & mmf.CELL_SOLVED) != 0).sum())
+    log("setup %.1fs: %d^3 cells (%d solved), %d interfaces, order %d"
+        % (time.perf_counter() - t_setup, n, solved, s.ninterfaces, args.order))
 
     s.advance(args.warmup)
     s.synchronize()
@@ -136,19 +170,24 @@ def main():
     if np_ or nr_:
         raise SystemExit("bench: negative pressure/density encountered (%d, %d)" % (np_, nr_))
 
-    value = N * 3 * args.steps / el
+    value = solved * 3 * args.steps / el
     avg_ms = stage_ms / max(launches, 1)
-    achieved = BYTES_PER_CELL_STAGE * N / (avg_ms * 1e-3) / 1e9
-    workload = "radial_sod_3d_%d_order%d" % (n, args.order)
+    bytes_per_cell = BYTES_PER_CELL_STAGE + mask_bytes
+    achieved = bytes_per_cell * N / (avg_ms * 1e-3) / 1e9
+    workload = "%s_%d_order%d" % (wname, n, args.order)
     log("timed %d steps in %.3f s: %.3f ms/step, stage kernel avg %.4f ms (%d launches), %.0f GB/s, t=%.5f dt=%.3e"
         % (args.steps, el, el / args.steps * 1e3, avg_ms, launches, achieved, t, dt))
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "radsod256":
         log("cpu baseline: oracle, %d step(s) at %d^3 ..." % (args.cpu_steps, n))
         cpu = cpu_baseline(n, args.cpu_steps)
         log("cpu baseline: %.3e cell-updates/s" % cpu["value"])
 
+    config = {"workload": workload, "cells": N, "solved_cells": solved, "interfaces": s.ninterfaces}
+    config.update(cfg)
+    config.update({"reconstruction": "order %d" % args.order, "flux": "Rusanov", "t_after_timed_steps": t,
+                   "parallelism": "single GPU"})
     out = {
         "metric": "cell-updates/s per RK stage",
         "value": value,
@@ -161,20 +200,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (radial Sod initial condition sampled at cell centroids)",
-        "config": {
-            "workload": workload,
-            "cells": N,
-            "interfaces": s.ninterfaces,
-            "domain": "[-1,1]^3",
-            "initial": "r<0.4: rho=1 p=1; else rho=0.125 p=0.1; u=0",
-            "boundaries": "transmissive (free flow)",
-            "reconstruction": "order %d" % args.order,
-            "flux": "Rusanov",
-            "time": "SSP-RK3, CFL 0.5, t_end 0.25",
-            "t_after_timed_steps": t,
-            "parallelism": "single GPU",
-        },
+        "data": "synthetic (initial condition sampled at cell centroids)",
+        "config": config,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -185,7 +212,7 @@ def main():
             "kernel": "%s (fused flux + RK stage), mean of stages 1-3" % (
                 "k_march tile 64x%d, %d planes/block" % (kinfo[1], kinfo[2]) if kinfo[0] == mmf.KERNEL_MARCH
                 else "k_stage (per-cell gather)"),
-            "algorithmic_bytes_per_launch": BYTES_PER_CELL_STAGE * N,
+            "algorithmic_bytes_per_launch": bytes_per_cell * N,
             "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,
