@@ -25,6 +25,7 @@ struct mmf_ctx {
   mmf_grid_desc desc;
   GridDev g;
   int device = 0;
+  int num_cus = 0;
   hipStream_t stream = nullptr;
   double *dU = nullptr, *dR = nullptr, *dW = nullptr;
   uint8_t *dflags = nullptr;
@@ -192,13 +193,14 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
     g.pow2A = std::frexp(g.A, &e2) == 0.5 ? 1 : 0;
     g.halfA = 0.5 * g.A;
   }
-  ctx->shape = mmf::default_march_shape(g);
-
   auto cleanup = [&](int code) {
     mmf_destroy(ctx);
     return code;
   };
   if (hipGetDevice(&ctx->device) != hipSuccess) return cleanup(MMF_ERR_NO_DEVICE);
+  if (hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+    ctx->num_cus = 0;
+  ctx->shape = mmf::default_march_shape(g, ctx->num_cus);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(MMF_ERR_HIP);
   const size_t state_bytes = sizeof(double) * 5 * (size_t)g.fs;
   if (hipMalloc(&ctx->dU, state_bytes) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
@@ -244,7 +246,7 @@ int mmf_destroy(mmf_ctx *ctx) {
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk) {
   if (!ctx) return MMF_ERR_INVALID;
   if (kind < mmf::KERNEL_AUTO || kind > mmf::KERNEL_MARCH) return fail(ctx, MMF_ERR_INVALID, "unknown kernel kind");
-  mmf::MarchShape sh = mmf::default_march_shape(ctx->g);
+  mmf::MarchShape sh = mmf::default_march_shape(ctx->g, ctx->num_cus);
   if (tile_rows > 0) sh.ty = tile_rows;
   if (chunk > 0) sh.chunk = chunk;
   if (kind == mmf::KERNEL_MARCH && !mmf::march_shape_supported(ctx->g, sh))

@@ -55,7 +55,7 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int sta
                               const double *R, const double *W, double *out_R, double *out_W, double *out_U,
                               Ctrl *ctrl, hipStream_t stream);
 bool march_shape_supported(const GridDev &g, const MarchShape &shape);
-MarchShape default_march_shape(const GridDev &g);
+MarchShape default_march_shape(const GridDev &g, int num_cus);
 hipError_t launch_advance(Ctrl *ctrl, hipStream_t stream);
 hipError_t launch_zero_slot(Ctrl *ctrl, int slot, hipStream_t stream);
 hipError_t launch_error_norms(const GridDev &g, const double *U, const double *rho_ex, double *partials,

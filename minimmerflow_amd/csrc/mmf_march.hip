@@ -538,15 +538,17 @@ bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
   return sh.ty == 4 || sh.ty == 8;
 }
 
-MarchShape default_march_shape(const GridDev &g) {
+// Longest chunk (fewest boundary planes re-read) that still gives every CU a
+// block: one 64x8-tile block per CU is resident at a time (LDS and VGPRs).
+MarchShape default_march_shape(const GridDev &g, int num_cus) {
   MarchShape sh;
-  if (g.dim == 2) {
-    sh.ty = 1;
-    sh.chunk = 16;
-  } else {
-    sh.ty = 8;
-    sh.chunk = 128;
-  }
+  sh.ty = (g.dim == 2) ? 1 : 8;
+  const int M = g.dim - 1;
+  const int64_t ntx = (g.n[0] + TX - 1) / TX;
+  const int64_t nty = (g.dim == 3) ? (g.n[1] + sh.ty - 1) / sh.ty : 1;
+  const int64_t want = (num_cus > 0) ? num_cus : 256;
+  sh.chunk = 128;
+  while (sh.chunk > 8 && ntx * nty * ((g.n[M] + sh.chunk - 1) / sh.chunk) < want) sh.chunk /= 2;
   return sh;
 }
 
