@@ -6,7 +6,8 @@ inside r<0.4 rho=1 p=1, outside rho=0.125 p=0.1, at rest, transmissive
 (free-flow) borders, first-order Rusanov + SSP-RK3 at CFL 0.5 towards t=0.25,
 FP64.  --workload sphere192 runs the immersed-sphere case instead (192^3 on
 [-2,2]^3, sphere r=0.25, Mach 0.3 free stream, Dirichlet inflow at x-,
-free-flow elsewhere).
+free-flow elsewhere); --workload batch512 runs 512 independent seeded 2D
+cases together (persistent stage kernel, graph-replayed step).
 
 A step is one SSP-RK3 time step (three fused flux+update stage kernels and
 the device-side dt/t update) over the whole grid, with the state resident
@@ -75,7 +76,8 @@ def sphere_setup(mmf, n, order):
     return s, pb, 1.0, cfg
 
 
-WORKLOADS = {"radsod256": (radsod_setup, 256, "radial_sod_3d"), "sphere192": (sphere_setup, 192, "sphere_3d")}
+WORKLOADS = {"radsod256": (radsod_setup, 256, "radial_sod_3d"), "sphere192": (sphere_setup, 192, "sphere_3d"),
+             "batch512": (None, 512, "batch_2d_mixed")}
 
 
 def cpu_baseline(n, steps):
@@ -102,6 +104,117 @@ def cpu_baseline(n, steps):
     }
 
 
+def cpu_baseline_batch(descs, problems, budget_s=12.0):
+    """The oracle (plain C, one thread) stepping the cases of the batch one
+    after another, from their initial conditions, until about budget_s."""
+    from oracle import oracle as O
+    keys = {0: O.VORTEX_XY, 3: O.RADSOD}
+    cells = 0
+    ncases = 0
+    el = 0.0
+    for d, pm in zip(descs, problems):
+        po = O.problem(keys[pm.type], 2, vortex_beta=pm.vortex_beta, vortex_center=list(pm.vortex_center),
+                       sod_p_in=pm.sod_p_in, sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out, sod_t_out=pm.sod_t_out)
+        n = [int(d.n[0]), int(d.n[1])]
+        s = O.Solver(po, 1, n, list(d.origin), d.h, side_bc=[d.side_bc[k] for k in range(6)])
+        U = s.initial_state()
+        W = U.copy()
+        t0 = time.perf_counter()
+        s.rk3_step(U, W, 0.5, 0.0, 1e30)
+        el += time.perf_counter() - t0
+        cells += n[0] * n[1]
+        ncases += 1
+        if el >= budget_s:
+            break
+    return {
+        "value": cells * 3 / el,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "one SSP-RK3 step of each of the first %d cases of the same 512-case set (%d cells), "
+                  "oracle/mmf_oracle.c -O2, single thread on %s; %.1f s" % (ncases, cells, cpu_model(), el),
+    }
+
+
+def run_batch(args, mmf):
+    """512 independent 2D cases advanced together: persistent stage kernel,
+    graph-replayed step, launch shape from the autotuner."""
+    from minimmerflow_amd import workloads as WL
+    t_setup = time.perf_counter()
+    descs, problems, params = WL.mixed_2d_cases(mmf, args.ncases, WL.DEFAULT_SEED)
+    b = mmf.Batch(descs)
+    b.set_initial_states(problems)
+    b.set_time(0.0, 1e30, 0.5)  # a fixed number of steps: every case stays active
+    N = b.total_cells
+    log("setup %.1fs: %d cases, %d cells, %d work items" % (time.perf_counter() - t_setup, b.ncases, N, b.nitems))
+    if args.chunk or args.blocks:
+        b.set_shape(args.chunk, args.blocks)
+        tuned = None
+    else:
+        chunk, blocks, ms = b.autotune(steps_per_trial=3)
+        tuned = {"chunk": chunk, "blocks": blocks, "ms_per_step": ms}
+        log("autotune: %d rows per item, %d persistent blocks (%.4f ms/step)" % (chunk, blocks, ms))
+    chunk, blocks = b.shape()
+    b.advance(args.warmup)
+    b.times()
+    b.enable_timing(True)
+    t0 = time.perf_counter()
+    b.advance(args.steps)
+    t, steps = b.times()
+    el = time.perf_counter() - t0
+    step_ms, nsteps = b.read_timing()
+    b.enable_timing(False)
+    if not (steps == args.warmup + args.steps).all():
+        raise SystemExit("bench: some cases did not take every step")
+    value = N * 3 * args.steps / el
+    avg_step_ms = step_ms / max(nsteps, 1)
+    achieved = BYTES_PER_CELL_STAGE * 3 * N / (avg_step_ms * 1e-3) / 1e9
+    workload = "batch_2d_mixed_%d_order1" % b.ncases
+    log("timed %d steps in %.3f s: %.3f ms/step (graph replay avg %.4f ms), %.0f GB/s"
+        % (args.steps, el, el / args.steps * 1e3, avg_step_ms, achieved))
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle over a sample of the cases ...")
+        cpu = cpu_baseline_batch(descs, problems)
+        log("cpu baseline: %.3e cell-updates/s" % cpu["value"])
+    config = {"workload": workload, "cases": b.ncases, "cells": N, "work_items": b.nitems,
+              "case_set": "SplitMix64 seed 0x6D6D66: side ~ {32,48,...,256}; even cases isentropic vortex "
+                          "(beta~U[2,6], centre~U[-2,2]^2, [-5,5]^2, free flow), odd cases radial Sod "
+                          "(rho and p ratios ~U[4,16], [0,8]^2, r0=1, reflecting)",
+              "sides": WL.side_histogram(params), "time": "SSP-RK3, CFL 0.5, fixed step count, per-case dt",
+              "shape": {"rows_per_item": chunk, "persistent_blocks": blocks, "autotuned": tuned},
+              "reconstruction": "order 1", "flux": "Rusanov", "parallelism": "single GPU"}
+    out = {
+        "metric": "cell-updates/s per RK stage",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded initial conditions sampled at cell centroids)",
+        "config": config,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(workload),
+            "kernel": "k_batch2d (persistent, fused flux + RK stage), whole graph-replayed step "
+                      "(three stage launches + per-case time update)",
+            "algorithmic_bytes_per_step": BYTES_PER_CELL_STAGE * 3 * N,
+            "avg_step_ms": avg_step_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if os.path.exists(p):
@@ -123,6 +236,8 @@ def main():
     ap.add_argument("--kernel", choices=("auto", "gather", "march"), default="auto")
     ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4 or 8)")
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
+    ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
+    ap.add_argument("--ncases", type=int, default=512, help="batch: number of cases")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -140,6 +255,10 @@ def main():
         ms = mmf.calibration_copy(args.copy_calibration, reps=5)
         print(json.dumps({"copy_bytes": args.copy_calibration, "ms_per_copy": ms,
                           "GBs_read_plus_write": 2 * args.copy_calibration / (ms * 1e-3) / 1e9}), flush=True)
+        return
+
+    if args.workload == "batch512":
+        run_batch(args, mmf)
         return
 
     setup, n_default, wname = WORKLOADS[args.workload]

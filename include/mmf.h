@@ -209,6 +209,48 @@ int mmf_error_vs_exact(mmf_ctx *ctx, const mmf_problem *pb, double t, double *l1
  * current host order */
 int mmf_get_centroids(const mmf_ctx *ctx, double *xyz);
 
+/* ---- batched mode: many independent 2D order-1 grids ----------------------- */
+/* The reference runs one grid per process; a batch advances ncases uneven 2D
+ * grids together, each with the reference's own time loop (main.cpp:386-512:
+ * its own dt from its own stage-1 maximum eigenvalue, its own end time).
+ * One step is three persistent stage launches plus the per-case time update,
+ * captured once into a HIP graph and replayed.  Cases are order 1, with no
+ * immersed bodies; border BCs as in the grid descriptor, periodic allowed.  Per
+ * case the results are bitwise those of a single-grid mmf_ctx. */
+typedef struct mmf_batch mmf_batch;
+
+int mmf_batch_create(const mmf_grid_desc *cases, int32_t ncases, mmf_batch **out);
+int mmf_batch_destroy(mmf_batch *b);
+const char *mmf_batch_last_error(const mmf_batch *b);
+/* total cells over all cases and the number of (case, strip, row chunk) work items */
+int mmf_batch_sizes(const mmf_batch *b, int64_t *total_cells, int32_t *nitems);
+/* conservative state of case ci, host AoS [ncells][5], cells x-fastest */
+int mmf_batch_set_state(mmf_batch *b, int32_t ci, const double *cons);
+int mmf_batch_get_state(mmf_batch *b, int32_t ci, double *cons);
+/* problems[ncases]: exact solution at t = 0 at the cell centroids (main.cpp:335-344) */
+int mmf_batch_set_initial_states(mmf_batch *b, const mmf_problem *problems);
+/* per-case start time, end time and CFL (arrays of ncases) */
+int mmf_batch_set_time(mmf_batch *b, const double *t0, const double *tmax, const double *cfl);
+/* rows per work item and persistent grid size (<= 0 keeps the current value) */
+int mmf_batch_set_shape(mmf_batch *b, int32_t chunk, int32_t blocks);
+int mmf_batch_get_shape(const mmf_batch *b, int32_t *chunk, int32_t *blocks);
+/* enqueue nsteps RK3 steps (graph replays); cases past their end time idle */
+int mmf_batch_advance(mmf_batch *b, int64_t nsteps);
+/* steps until every case reaches its end time (max_steps < 0: no limit) */
+int mmf_batch_run(mmf_batch *b, int64_t max_steps, int64_t *steps_done);
+/* per-case time and completed step count (synchronises); either may be NULL */
+int mmf_batch_get_times(mmf_batch *b, double *t, int64_t *steps);
+/* main.cpp:550-561 per case against problems[i] at the case's current time */
+int mmf_batch_error_vs_exact(mmf_batch *b, const mmf_problem *problems, double *l1, double *l2);
+/* graph-replay timing with HIP events (whole step: three stages + update) */
+int mmf_batch_enable_timing(mmf_batch *b, int32_t on);
+int mmf_batch_read_timing(mmf_batch *b, double *ms, int64_t *steps);
+/* Launch-shape tuner: times candidate (chunk, blocks) pairs over
+ * steps_per_trial steps each on a snapshot of the state, restores the state
+ * and keeps the fastest.  Results do not depend on the shape. */
+int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chunk, int32_t *best_blocks,
+                       double *best_ms_per_step);
+
 /* ---- bandwidth calibration ------------------------------------------------ */
 /* Copies nbytes device-to-device with a plain vector-load/store kernel reps
  * times; returns the average kernel milliseconds (HIP events). */

@@ -8,6 +8,7 @@ The classes mirror the reference's entry points:
   Solver.compute_rhs   -> euler::computeRHS            (euler.cpp:127-249)
   Solver.advance/run   -> the RK3 loop of computation() (main.cpp:374-506)
   Solver.set_bodies    -> body::isPointFluid + flags   (body.cpp:80-95, main.cpp:222-277)
+  Batch                -> many independent 2D runs of computation() advanced together
   Problem              -> problem::*                   (problem.cpp)
   batch_* functions    -> utils::* / euler::eval*      (utils.cpp, euler.cpp)
 """
@@ -120,6 +121,23 @@ def lib():
         "mmf_calibration_copy": ([C.c_int64, C.c_int32, dp], C.c_int),
         "mmf_set_kernel": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "mmf_get_kernel": ([vp, i32p, i32p, i32p], C.c_int),
+        "mmf_batch_create": ([C.POINTER(GridDesc), C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
+        "mmf_batch_destroy": ([vp], C.c_int),
+        "mmf_batch_last_error": ([vp], C.c_char_p),
+        "mmf_batch_sizes": ([vp, i64p, i32p], C.c_int),
+        "mmf_batch_set_state": ([vp, C.c_int32, dp], C.c_int),
+        "mmf_batch_get_state": ([vp, C.c_int32, dp], C.c_int),
+        "mmf_batch_set_initial_states": ([vp, C.POINTER(Problem)], C.c_int),
+        "mmf_batch_set_time": ([vp, dp, dp, dp], C.c_int),
+        "mmf_batch_set_shape": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "mmf_batch_get_shape": ([vp, i32p, i32p], C.c_int),
+        "mmf_batch_advance": ([vp, C.c_int64], C.c_int),
+        "mmf_batch_run": ([vp, C.c_int64, i64p], C.c_int),
+        "mmf_batch_get_times": ([vp, dp, i64p], C.c_int),
+        "mmf_batch_error_vs_exact": ([vp, C.POINTER(Problem), dp, dp], C.c_int),
+        "mmf_batch_enable_timing": ([vp, C.c_int32], C.c_int),
+        "mmf_batch_read_timing": ([vp, dp, i64p], C.c_int),
+        "mmf_batch_autotune": ([vp, C.c_int32, i32p, i32p, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -409,3 +427,132 @@ class Solver:
         ms, n = C.c_double(), C.c_int64()
         self._c(lib().mmf_read_timing(self.ctx, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+# ---- batched mode: many independent 2D grids -------------------------------------
+
+def grid_desc(dim, n, origin, h, side_bc, side_info=None, order=1):
+    d = GridDesc()
+    d.dim, d.order, d.h = int(dim), int(order), float(h)
+    n = list(n) + [1] * (3 - len(n))
+    for a in range(3):
+        d.n[a] = int(n[a])
+        d.origin[a] = float(origin[a]) if a < len(origin) else 0.0
+    for s in range(6):
+        d.side_bc[s] = int(side_bc[s]) if s < len(side_bc) else BC_FREE_FLOW
+        if side_info is not None:
+            for k in range(NF):
+                d.side_info[s][k] = float(side_info[s][k])
+    return d
+
+
+def problem_grid_desc(pb, ncells, order=1):
+    """Grid of computation() for a problem (main.cpp:147-149, 251-260)."""
+    d = GridDesc()
+    d.dim, d.order, d.h = pb.dim, int(order), pb.domain_length / int(ncells)
+    for a in range(3):
+        d.n[a] = int(ncells) if a < pb.dim else 1
+        d.origin[a] = pb.domain_origin[a]
+    _check(lib().mmf_problem_side_bcs(C.byref(pb), C.byref(d)))
+    return d
+
+
+class Batch:
+    """Many independent 2D order-1 grids advanced together on the device.
+
+    Each case keeps the reference's own time loop (own dt, own end time); one
+    step of the whole batch is three persistent stage launches plus the time
+    update, replayed from a captured HIP graph.  Per case the results are
+    bitwise those of a single-grid Solver."""
+
+    def __init__(self, descs):
+        self.ncases = len(descs)
+        arr = (GridDesc * self.ncases)(*descs)
+        self.descs = list(descs)
+        self.ncells = [int(d.n[0] * d.n[1]) for d in descs]
+        h = C.c_void_p()
+        _check(lib().mmf_batch_create(arr, self.ncases, C.byref(h)))
+        self.h = h
+        tc, ni = C.c_int64(), C.c_int32()
+        self._c(lib().mmf_batch_sizes(self.h, C.byref(tc), C.byref(ni)))
+        self.total_cells, self.nitems = tc.value, ni.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mmf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        if rc != OK:
+            m = lib().mmf_batch_last_error(self.h) if getattr(self, "h", None) else None
+            raise MMFError(rc, m.decode() if m else "")
+
+    def set_state(self, i, U):
+        U = _f64(U).reshape(self.ncells[i], NF)
+        self._c(lib().mmf_batch_set_state(self.h, int(i), _dp(U)))
+
+    def get_state(self, i):
+        U = np.empty((self.ncells[i], NF))
+        self._c(lib().mmf_batch_get_state(self.h, int(i), _dp(U)))
+        return U
+
+    def set_initial_states(self, problems):
+        arr = (Problem * self.ncases)(*problems)
+        self._c(lib().mmf_batch_set_initial_states(self.h, arr))
+
+    def set_time(self, t0, tmax, cfl):
+        t0 = _f64(np.broadcast_to(t0, (self.ncases,)))
+        tm = _f64(np.broadcast_to(tmax, (self.ncases,)))
+        cf = _f64(np.broadcast_to(cfl, (self.ncases,)))
+        self._c(lib().mmf_batch_set_time(self.h, _dp(t0), _dp(tm), _dp(cf)))
+
+    def set_shape(self, chunk=0, blocks=0):
+        self._c(lib().mmf_batch_set_shape(self.h, int(chunk), int(blocks)))
+        tc, ni = C.c_int64(), C.c_int32()
+        self._c(lib().mmf_batch_sizes(self.h, C.byref(tc), C.byref(ni)))
+        self.nitems = ni.value
+
+    def shape(self):
+        a, b = C.c_int32(), C.c_int32()
+        self._c(lib().mmf_batch_get_shape(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def advance(self, nsteps):
+        self._c(lib().mmf_batch_advance(self.h, int(nsteps)))
+
+    def run(self, max_steps=-1):
+        n = C.c_int64()
+        self._c(lib().mmf_batch_run(self.h, int(max_steps), C.byref(n)))
+        return n.value
+
+    def times(self):
+        t = np.empty(self.ncases)
+        st = np.empty(self.ncases, dtype=np.int64)
+        self._c(lib().mmf_batch_get_times(self.h, _dp(t), st.ctypes.data_as(C.POINTER(C.c_int64))))
+        return t, st
+
+    def error_vs_exact(self, problems):
+        arr = (Problem * self.ncases)(*problems)
+        l1, l2 = np.empty(self.ncases), np.empty(self.ncases)
+        self._c(lib().mmf_batch_error_vs_exact(self.h, arr, _dp(l1), _dp(l2)))
+        return l1, l2
+
+    def enable_timing(self, on=True):
+        self._c(lib().mmf_batch_enable_timing(self.h, 1 if on else 0))
+
+    def read_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        self._c(lib().mmf_batch_read_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def autotune(self, steps_per_trial=3):
+        a, b, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._c(lib().mmf_batch_autotune(self.h, int(steps_per_trial), C.byref(a), C.byref(b), C.byref(ms)))
+        self.set_shape(0, 0)
+        return a.value, b.value, ms.value

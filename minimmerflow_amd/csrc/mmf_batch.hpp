@@ -1,0 +1,28 @@
+// mmf_batch.hpp -- device structures of the batched (many independent grids) mode.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mmf_internal.hpp"
+
+namespace mmf {
+
+// One independent case of a batch: its grid, its own time-control block, where
+// its SoA state starts in the batch's shared state buffers, and its tiling.
+struct BatchCase {
+  GridDev g;       // g.fs is this case's field stride
+  Ctrl *ctrl;      // device pointer into the batch's control array
+  int64_t offset;  // first double of this case in U / R / W
+  int32_t ntx;     // strips along x
+  int32_t chunk;   // rows per item along y
+};
+
+hipError_t launch_stage_march2d(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
+                                const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s);
+int march2d_tiles(int64_t nx);
+hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,
+                              double *U, double *R, double *W, int nblocks, hipStream_t s);
+hipError_t launch_batch_advance(const BatchCase *cases, int ncases, unsigned int *counters, hipStream_t s);
+
+}  // namespace mmf

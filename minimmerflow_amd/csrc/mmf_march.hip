@@ -26,6 +26,7 @@
 #include "mmf_device.hpp"
 #include "mmf_internal.hpp"
 #include "mmf_stage.hpp"
+#include "mmf_batch.hpp"
 
 namespace mmf {
 
@@ -544,9 +545,10 @@ MarchShape default_march_shape(const GridDev &g, int num_cus) {
   MarchShape sh;
   sh.ty = (g.dim == 2) ? 1 : 8;
   const int M = g.dim - 1;
-  const int64_t ntx = (g.n[0] + TX - 1) / TX;
+  // 2D strips are one wave, 62 interior cells wide; several fit on a CU
+  const int64_t ntx = (g.dim == 2) ? (g.n[0] + 61) / 62 : (g.n[0] + TX - 1) / TX;
   const int64_t nty = (g.dim == 3) ? (g.n[1] + sh.ty - 1) / sh.ty : 1;
-  const int64_t want = (num_cus > 0) ? num_cus : 256;
+  const int64_t want = ((num_cus > 0) ? num_cus : 256) * ((g.dim == 2) ? 4 : 1);
   sh.chunk = 128;
   while (sh.chunk > 8 && ntx * nty * ((g.n[M] + sh.chunk - 1) / sh.chunk) < want) sh.chunk /= 2;
   return sh;
@@ -556,10 +558,7 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &sh, int stage,
                               const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                               hipStream_t s) {
   const bool mask = g.flags != nullptr;
-  if (g.dim == 2) {
-    return mask ? launch_march_s<2, 1, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
-                : launch_march_s<2, 1, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-  }
+  if (g.dim == 2) return launch_stage_march2d(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   switch (sh.ty) {
     case 4:
       return mask ? launch_march_s<3, 4, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)

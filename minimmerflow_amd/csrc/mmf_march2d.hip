@@ -1,0 +1,492 @@
+// mmf_march2d.hip -- 2D order-1 stage kernels: single grid and persistent batch.
+//
+// One wave owns an item: a strip of 62 cells along x plus one halo cell on
+// each side (lanes 0 and 63; x0 = 62 * tile), marching along y over a chunk
+// of rows.  Every lane builds exactly one flux tuple per row (interior or
+// halo), so no lane does a second pass.  Per row:
+//   1. take the row's inputs (loaded one row ahead) and build the tuple; put
+//      the next row's loads in flight; store the tuple in LDS (two buffers);
+//   2. evaluate the y face to the previous row from registers (each y face
+//      once), which completes the previous cell: write its RK stage update;
+//   3. after a wave barrier, evaluate the two x faces from LDS neighbours.
+// Each cell's residual is summed in the reference's face order (x lower, x
+// upper, y lower, y upper), bit-identical to the gather kernel and the oracle.
+//
+// The batch kernel is persistent: a fixed grid of one-wave blocks pulls
+// (case, tile, chunk) items from an atomic counter until the list is
+// exhausted; each case has its own grid, time-control block and dt.
+#include <hip/hip_runtime.h>
+
+#include "mmf_batch.hpp"
+#include "mmf_device.hpp"
+#include "mmf_internal.hpp"
+#include "mmf_stage.hpp"
+
+namespace mmf {
+
+namespace {
+
+constexpr int NTUP = 11;
+constexpr int W2 = 64;        // lanes
+constexpr int INNER = W2 - 2;  // interior cells per strip
+
+__device__ __forceinline__ void tstore(double *L, int slot, const Tup &t) {
+#pragma unroll
+  for (int k = 0; k < NF; ++k) L[k * W2 + slot] = t.c[k];
+  L[5 * W2 + slot] = t.vel[0];
+  L[6 * W2 + slot] = t.vel[1];
+  L[7 * W2 + slot] = t.vel[2];
+  L[8 * W2 + slot] = t.p;
+  L[9 * W2 + slot] = t.a;
+  L[10 * W2 + slot] = t.H;
+}
+
+__device__ __forceinline__ void tload(const double *L, int slot, Tup &t) {
+#pragma unroll
+  for (int k = 0; k < NF; ++k) t.c[k] = L[k * W2 + slot];
+  t.vel[0] = L[5 * W2 + slot];
+  t.vel[1] = L[6 * W2 + slot];
+  t.vel[2] = L[7 * W2 + slot];
+  t.p = L[8 * W2 + slot];
+  t.a = L[9 * W2 + slot];
+  t.H = L[10 * W2 + slot];
+}
+
+__device__ __forceinline__ double ld(const double *base, int64_t fs, int k, uint32_t i) { return base[k * fs + i]; }
+
+template <int STAGE, bool P2>
+struct In2 {
+  const double *X, *U, *R;
+  int64_t fs;
+  __device__ __forceinline__ void state(const GridDev &g, double dt, uint32_t i, double *c) const {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) c[k] = (STAGE == 2) ? ld(U, fs, k, i) + div_vol(g, dt * ld(R, fs, k, i)) : ld(X, fs, k, i);
+  }
+};
+
+template <int STAGE>
+struct Pf2 {
+  double a[NF], b[NF];
+  template <bool P2>
+  __device__ __forceinline__ void issue(const In2<STAGE, P2> &in, uint32_t i) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      if (STAGE == 2) {
+        a[k] = ld(in.U, in.fs, k, i);
+        b[k] = ld(in.R, in.fs, k, i);
+      } else {
+        a[k] = ld(in.X, in.fs, k, i);
+      }
+    }
+  }
+  __device__ __forceinline__ void state(const GridDev &g, double dt, double *X) const {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) X[k] = (STAGE == 2) ? a[k] + div_vol(g, dt * b[k]) : a[k];
+  }
+};
+
+__device__ __forceinline__ double vface(const GridDev &g, bool wall, int side, int a, const Tup &Fl, bool fluid_lower,
+                                        double *AF, Ctrl *ctrl, bool count) {
+  Tup V;
+  if (!wall && g.bc[side] == BC_FREE_FLOW) {
+    V = Fl;
+  } else {
+    double v[NF];
+    if (wall) reflect_axis(Fl.c, a, v);
+    else border_state(g, side, a, Fl.c, v);
+    make_tup(v, V);
+  }
+  double lam;
+  if (fluid_lower) {
+    lam = face_flux_area(Fl, V, a, g.pow2A, g.A, g.halfA, AF);
+    if (count) count_neg(Fl, V, ctrl);
+  } else {
+    lam = face_flux_area(V, Fl, a, g.pow2A, g.A, g.halfA, AF);
+    if (count) count_neg(V, Fl, ctrl);
+  }
+  return lam;
+}
+
+// One item: strip 'tile' of rows [m0, m1) of grid g.  U/out are the state
+// bases of this grid.  Returns the item's maximum eigenvalue.
+template <int STAGE, bool MASK, bool P2>
+__device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, const double *Ub, double *outb, double dt,
+                               Ctrl *ctrl, int tile, int m0, int m1, double *ldsb, uint8_t *lflb) {
+  GridDev g = g0;
+  g.pow2A = P2;
+  g.pow2V = P2;
+  const int lane = threadIdx.x;
+  const int nx = (int)g.n[0];
+  const int nm = (int)g.n[1];
+  const int64_t fs = in.fs;
+  const int x0 = tile * INNER;
+  const int w = (nx - x0 < INNER) ? nx - x0 : INNER;
+  const int gx = x0 - 1 + lane;
+  const bool valid = lane >= 1 && lane <= w;
+  bool halo = (lane == 0) || (lane == w + 1);
+  int cx = gx;
+  if (halo && (cx < 0 || cx >= nx)) {
+    if (g.periodic[0]) cx = (cx + nx) % nx;
+    else halo = false;
+  }
+  const bool has_tuple = valid || halo;
+  const uint32_t col = (uint32_t)cx;
+  const uint32_t sm = (uint32_t)nx;
+  const bool has_xm = valid && (gx > 0 || g.periodic[0]);
+  const bool has_xp = valid && (gx + 1 < nx || g.periodic[0]);
+  const bool top_wrap_here = g.periodic[1] && m1 == nm;
+  const double A = g.A, halfA = g.halfA;
+
+  double lam_max = 0.0;
+  Tup Tp;
+  uint8_t fprev = CELL_FLUID | CELL_SOLVED;
+  bool bad_prev = false;
+  double part[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) part[k] = 0.0;
+
+  {
+    int mprev = m0 - 1;
+    bool have_prev = true;
+    if (mprev < 0) {
+      if (g.periodic[1]) mprev += nm;
+      else have_prev = false;
+    }
+    if (valid && have_prev) {
+      double X[NF];
+      in.state(g, dt, col + (uint32_t)mprev * sm, X);
+      make_tup(X, Tp);
+      fprev = cell_flags<MASK>(g, col + (uint32_t)mprev * sm);
+      bad_prev = tup_negative(Tp);
+    }
+  }
+  const bool have_prev0 = (m0 > 0) || g.periodic[1];
+
+  Pf2<STAGE> pf;
+  if (has_tuple) pf.issue(in, col + (uint32_t)m0 * sm);
+  double Uf[NF], Ucur[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k] = 0.0;
+
+  int buf = 0;
+  for (int m = m0; m < m1; ++m) {
+    const uint32_t idx = col + (uint32_t)m * sm;
+    double *L = ldsb + buf * (NTUP * W2);
+    uint8_t *LF = lflb + (MASK ? buf * W2 : 0);
+    Tup Tc;
+    uint8_t fc = CELL_FLUID | CELL_SOLVED;
+    bool bad_c = false;
+    double X[NF];
+    if (has_tuple) {
+      pf.state(g, dt, X);
+      if (valid && STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = pf.a[k];
+      } else if (valid && STAGE == 3) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = ld(Ub, fs, k, idx);
+      }
+      if (m + 1 < m1) pf.issue(in, idx + sm);
+      make_tup(X, Tc);
+      bad_c = tup_negative(Tc);
+      fc = cell_flags<MASK>(g, idx);
+      tstore(L, lane, Tc);
+      if (MASK) LF[lane] = fc;
+    }
+
+    double AFy[NF];
+    bool yact = false;
+    if (valid) {
+      const bool hl = (m > m0) || have_prev0;
+      if (!hl) {
+        yact = fc & CELL_SOLVED;
+        if (yact) {
+          double l = vface(g, false, 2, 1, Tc, false, AFy, ctrl, true);
+          lam_max = (l < lam_max) ? lam_max : l;
+        }
+      } else {
+        yact = (fprev | fc) & CELL_SOLVED;
+        if (yact) {
+          double l;
+          if (MASK && ((fprev ^ fc) & CELL_FLUID)) {
+            if (fc & CELL_FLUID) l = vface(g, true, 2, 1, Tc, false, AFy, ctrl, true);
+            else l = vface(g, true, 3, 1, Tp, true, AFy, ctrl, true);
+          } else {
+            l = face_flux_area(Tp, Tc, 1, P2, A, halfA, AFy);
+            if (bad_prev | bad_c) count_neg(Tp, Tc, ctrl);
+          }
+          lam_max = (l < lam_max) ? lam_max : l;
+        }
+      }
+      if (m > m0) {
+        const uint32_t pidx = idx - sm;
+        if (fprev & CELL_SOLVED) {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFy[k], dt);
+        } else if (STAGE != 3) {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+        }
+      }
+      if (STAGE >= 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k];
+      }
+    }
+    __syncthreads();
+
+    if (valid) {
+      if (fc & CELL_SOLVED) {
+        double acc[NF] = {0., 0., 0., 0., 0.};
+        double AFm[NF], AFp[NF];
+        {
+          const uint8_t fm = MASK && has_xm ? LF[lane - 1] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
+          const bool wall = MASK && has_xm && ((fm ^ fc) & CELL_FLUID);
+          double l;
+          if (has_xm && !wall) {
+            Tup Nb;
+            tload(L, lane - 1, Nb);
+            l = face_flux_area(Nb, Tc, 0, P2, A, halfA, AFm);
+            if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
+          } else {
+            l = vface(g, wall, 0, 0, Tc, false, AFm, ctrl, true);
+          }
+          lam_max = (l < lam_max) ? lam_max : l;
+        }
+        {
+          const uint8_t fp = MASK && has_xp ? LF[lane + 1] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
+          const bool wall = MASK && has_xp && ((fp ^ fc) & CELL_FLUID);
+          double l;
+          if (has_xp && !wall) {
+            Tup Nb;
+            tload(L, lane + 1, Nb);
+            l = face_flux_area(Tc, Nb, 0, P2, A, halfA, AFp);
+            if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
+          } else {
+            l = vface(g, wall, 1, 0, Tc, true, AFp, ctrl, true);
+          }
+          lam_max = (l < lam_max) ? lam_max : l;
+        }
+        // first two contributions from zero: their order does not change the bits
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          acc[k] += AFm[k];
+          acc[k] -= AFp[k];
+        }
+        if (top_wrap_here && m == nm - 1) {
+          double X0[NF], AFw[NF];
+          Tup T0;
+          in.state(g, dt, col, X0);
+          make_tup(X0, T0);
+          const uint8_t f0 = cell_flags<MASK>(g, col);
+          double l;
+          if (MASK && ((f0 ^ fc) & CELL_FLUID)) l = vface(g, true, 3, 1, Tc, true, AFw, ctrl, false);
+          else l = face_flux_area(Tc, T0, 1, P2, A, halfA, AFw);
+          lam_max = (l < lam_max) ? lam_max : l;
+#pragma unroll
+          for (int k = 0; k < NF; ++k)
+            outb[k * fs + idx] = rk_update<STAGE>(g, Uf[k], Tc.c[k], (acc[k] - AFw[k]) + (yact ? AFy[k] : 0.0), dt);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) part[k] = acc[k] + (yact ? AFy[k] : 0.0);
+        }
+      }
+      Tp = Tc;
+      fprev = fc;
+      bad_prev = bad_c;
+    }
+    buf ^= 1;
+  }
+
+  if (valid) {
+    const uint32_t pidx = col + (uint32_t)(m1 - 1) * sm;
+    if (top_wrap_here) {
+      if (!(fprev & CELL_SOLVED) && STAGE != 3) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+      }
+    } else {
+      const bool have_next = m1 < nm;
+      double AFy[NF];
+      bool act;
+      double l = 0.0;
+      if (have_next) {
+        double X[NF];
+        Tup Tn;
+        const uint32_t nidx = col + (uint32_t)m1 * sm;
+        in.state(g, dt, nidx, X);
+        make_tup(X, Tn);
+        const uint8_t fn = cell_flags<MASK>(g, nidx);
+        act = (fprev | fn) & CELL_SOLVED;
+        if (act) {
+          if (MASK && ((fprev ^ fn) & CELL_FLUID)) {
+            if (fn & CELL_FLUID) l = vface(g, true, 2, 1, Tn, false, AFy, ctrl, false);
+            else l = vface(g, true, 3, 1, Tp, true, AFy, ctrl, false);
+          } else {
+            l = face_flux_area(Tp, Tn, 1, P2, A, halfA, AFy);
+          }
+        }
+      } else {
+        act = fprev & CELL_SOLVED;
+        if (act) l = vface(g, false, 3, 1, Tp, true, AFy, ctrl, true);
+      }
+      if (act) lam_max = (l < lam_max) ? lam_max : l;
+      if (fprev & CELL_SOLVED) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFy[k], dt);
+      } else if (STAGE != 3) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+      }
+    }
+  }
+  return lam_max;
+}
+
+// wave-wide max of non-negative doubles as u64 bits; lane 0 holds the result
+__device__ __forceinline__ unsigned long long wave_max_bits(double v) {
+  unsigned long long b = dbits(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off, 64);
+    b = (o > b) ? o : b;
+  }
+  return b;
+}
+
+}  // namespace
+
+// ---- single grid -------------------------------------------------------------
+template <int STAGE, bool MASK, bool P2>
+__global__ __launch_bounds__(W2) void k_march2d(GridDev g, const double *X, const double *Uin, const double *R,
+                                                double *out, Ctrl *ctrl, int mode, int chunk, int ntx) {
+  __shared__ double lds[2 * NTUP * W2];
+  __shared__ uint8_t lfl[MASK ? 2 * W2 : 1];
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;
+  const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+  const int tile = blockIdx.x % ntx;
+  const int bz = blockIdx.x / ntx;
+  const int nm = (int)g.n[1];
+  const int m0 = bz * chunk;
+  const int m1 = (m0 + chunk < nm) ? m0 + chunk : nm;
+  In2<STAGE, P2> in{X, Uin, R, g.fs};
+  double lam = march2d_item<STAGE, MASK, P2>(g, in, Uin, out, dt, ctrl, tile, m0, m1, lds, lfl);
+  unsigned long long b = wave_max_bits(lam);
+  if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[(mode == MODE_RHS_ONLY) ? 3 : STAGE - 1], b);
+}
+
+template <int STAGE, bool MASK, bool P2>
+static hipError_t launch2d_t(const GridDev &g, int chunk, int mode, const double *U, const double *R, const double *W,
+                             double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  const double *X = (STAGE == 3) ? W : U;
+  double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
+  const int ntx = (int)((g.n[0] + INNER - 1) / INNER);
+  const int nch = (int)((g.n[1] + chunk - 1) / chunk);
+  hipLaunchKernelGGL((k_march2d<STAGE, MASK, P2>), dim3((unsigned)(ntx * nch)), dim3(W2), 0, s, g, X, U, R, out,
+                     ctrl, mode, chunk, ntx);
+  return hipGetLastError();
+}
+
+template <bool MASK, bool P2>
+static hipError_t launch2d_s(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
+                             const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  switch (stage) {
+    case 1: return launch2d_t<1, MASK, P2>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return launch2d_t<2, MASK, P2>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return launch2d_t<3, MASK, P2>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
+}
+
+hipError_t launch_stage_march2d(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
+                                const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  const bool mask = g.flags != nullptr;
+  const bool p2 = g.pow2A && g.pow2V;
+  if (mask) return p2 ? launch2d_s<true, true>(g, chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                      : launch2d_s<true, false>(g, chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  return p2 ? launch2d_s<false, true>(g, chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+            : launch2d_s<false, false>(g, chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+}
+
+int march2d_tiles(int64_t nx) { return (int)((nx + INNER - 1) / INNER); }
+
+// ---- persistent batch ---------------------------------------------------------
+template <int STAGE>
+__global__ __launch_bounds__(W2) void k_batch2d(const BatchCase *cases, const int *items, int nitems,
+                                                unsigned int *counter, double *U, double *R, double *W) {
+  __shared__ double lds[2 * NTUP * W2];
+  __shared__ uint8_t lfl[1];
+  __shared__ int item_s;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = item_s;
+    __syncthreads();
+    if (it >= nitems) break;  // every wave of the block reaches this exit
+    const int ci = items[2 * it];
+    const int code = items[2 * it + 1];
+    const BatchCase &bc = cases[ci];
+    Ctrl *ctrl = bc.ctrl;
+    if (!(ctrl->t < ctrl->tmax)) continue;  // this case has reached its end time
+    const GridDev &g = bc.g;
+    const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+    const int tile = code % bc.ntx;
+    const int bz = code / bc.ntx;
+    const int nm = (int)g.n[1];
+    const int m0 = bz * bc.chunk;
+    const int m1 = (m0 + bc.chunk < nm) ? m0 + bc.chunk : nm;
+    double *Ub = U + bc.offset, *Rb = R + bc.offset, *Wb = W + bc.offset;
+    const double *X = (STAGE == 3) ? Wb : Ub;
+    double *out = (STAGE == 1) ? Rb : (STAGE == 2) ? Wb : Ub;
+    // batch cases carry no body mask (all cells fluid and solved)
+    double lam;
+    if (g.pow2A && g.pow2V)
+      lam = march2d_item<STAGE, false, true>(g, In2<STAGE, true>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds, lfl);
+    else
+      lam = march2d_item<STAGE, false, false>(g, In2<STAGE, false>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds, lfl);
+    unsigned long long b = wave_max_bits(lam);
+    if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[STAGE - 1], b);
+  }
+}
+
+// t += dt per case (main.cpp:505-506), resets the stage maxima and the item counters.
+__global__ void k_batch_advance(const BatchCase *cases, int ncases, unsigned int *counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 && blockIdx.x == 0) counters[i] = 0u;
+  if (i >= ncases) return;
+  Ctrl *c = cases[i].ctrl;
+  if (c->t < c->tmax) {
+    double dt = step_dt(c);
+    c->dt = dt;
+    c->t += dt;
+    c->step += 1;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) c->last_eig[s] = __longlong_as_double((long long)c->eig[s]);
+  }
+  c->eig[0] = 0ull;
+  c->eig[1] = 0ull;
+  c->eig[2] = 0ull;
+}
+
+hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,
+                              double *U, double *R, double *W, int nblocks, hipStream_t s) {
+  switch (stage) {
+    case 1:
+      hipLaunchKernelGGL(k_batch2d<1>, dim3(nblocks), dim3(W2), 0, s, cases, items, nitems, counters + 0, U, R, W);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_batch2d<2>, dim3(nblocks), dim3(W2), 0, s, cases, items, nitems, counters + 1, U, R, W);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch2d<3>, dim3(nblocks), dim3(W2), 0, s, cases, items, nitems, counters + 2, U, R, W);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(const BatchCase *cases, int ncases, unsigned int *counters, hipStream_t s) {
+  const int nb = (ncases + 255) / 256;
+  hipLaunchKernelGGL(k_batch_advance, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, cases, ncases, counters);
+  return hipGetLastError();
+}
+
+}  // namespace mmf

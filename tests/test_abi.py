@@ -42,3 +42,18 @@ def test_problem_catalogue_host_side(mmf, orc):
                 got = mmf.exact_conservatives(pm, pts, t)
                 exp = np.array([orc.exact_conservatives(po, p, t) for p in pts])
                 assert np.array_equal(got, exp), (key, dim, t)
+
+
+def test_batch_create_rejects_bad_cases_without_touching_a_device(mmf):
+    """Case validation runs on the host before any device call."""
+    import pytest
+    ok = mmf.grid_desc(2, [8, 8], [0.0, 0.0, 0.0], 0.125, [0, 0, 0, 0])
+    bad_dim = mmf.grid_desc(3, [8, 8, 8], [0.0, 0.0, 0.0], 0.125, [0] * 6)
+    bad_order = mmf.grid_desc(2, [8, 8], [0.0, 0.0, 0.0], 0.125, [0] * 4, order=2)
+    half_periodic = mmf.grid_desc(2, [8, 8], [0.0, 0.0, 0.0], 0.125, [4, 0, 0, 0])
+    bad_h = mmf.grid_desc(2, [8, 8], [0.0, 0.0, 0.0], 0.0, [0] * 4)
+    for descs, code in (([ok, bad_dim], mmf.ERR_INVALID), ([bad_order], mmf.ERR_UNSUPPORTED_ORDER),
+                        ([ok, half_periodic], mmf.ERR_INVALID), ([bad_h], mmf.ERR_INVALID), ([], mmf.ERR_INVALID)):
+        with pytest.raises(mmf.MMFError) as e:
+            mmf.Batch(descs)
+        assert e.value.status == code
