@@ -251,6 +251,82 @@ int mmf_batch_read_timing(mmf_batch *b, double *ms, int64_t *steps);
 int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chunk, int32_t *best_blocks,
                        double *best_ms_per_step);
 
+/* ---- learned face reconstruction (2D, order 2) ----------------------------- */
+/* A second form beside the plain drop-in, with no counterpart in the
+ * reference: the face values of the limited linear polynomial
+ * (reconstruction.cpp:67-97) are replaced by the output of a small network
+ * (5x5 stencil of rho, rho u, rho v, rho E, normalised per cell -> 64 -> 64
+ * (GELU) -> 16 outputs z), run in bf16 on the matrix cores with fp32
+ * accumulation inside the reconstruction kernel.  The network is a learned
+ * limiter: per face and component the value is P + sigmoid(z) (Q - P), P the
+ * minmod face value and Q the fifth-order upwind-biased linear value along
+ * the face's axis, so every learned value is limited and upwind.  The plain
+ * face values remain wherever the stencil crosses a non-periodic border, the
+ * shock sensor (normalised second difference of rho or p) exceeds its
+ * threshold, or a learned face state has non-positive density or pressure.
+ * Training (fp32 forward, backward and Adam kernels) uses rows built on the
+ * device from fine runs restricted by conservative averaging; targets are
+ * the fine plain face values averaged over each coarse face. */
+#define MMF_LEARN_NPARAMS 11664
+#define MMF_LEARN_NIN 100
+#define MMF_LEARN_NOUT 16
+typedef struct mmf_learn mmf_learn;
+
+/* network with U(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights from SplitMix64(seed),
+ * zero biases, and room for 'capacity' training rows */
+int mmf_learn_create(uint64_t seed, int64_t capacity, mmf_learn **out);
+int mmf_learn_destroy(mmf_learn *net);
+const char *mmf_learn_last_error(const mmf_learn *net);
+/* parameters W1[64][100], b1[64], W2[64][64], b2[64], W3[16][64], b3[16] (fp32);
+ * setting them resets the Adam moments */
+int mmf_learn_get_params(const mmf_learn *net, float *params);
+int mmf_learn_set_params(mmf_learn *net, const float *params);
+/* appends 8 rows (the dihedral images) per listed coarse cell of the fine
+ * context's current state restricted by 'factor' (1..8); rows whose stencil
+ * crosses a non-periodic border or whose sensor exceeds the threshold get
+ * weight 0 */
+int mmf_learn_collect(mmf_learn *net, mmf_ctx *fine, int32_t factor, const int64_t *coarse_cells, int32_t ncells,
+                      double sensor_threshold);
+int mmf_learn_num_samples(const mmf_learn *net, int64_t *count);
+/* rows x[count][100], targets t[count][16], weights w[count]; any may be NULL on get */
+int mmf_learn_get_samples(const mmf_learn *net, int64_t first, int64_t count, float *x, float *t, float *w);
+int mmf_learn_set_samples(mmf_learn *net, const float *x, const float *t, const float *w, int64_t count);
+/* loss sum w (yhat - t)^2 / (16 count), yhat = P + sigmoid(z) (Q - P) in
+ * normalised units, and its gradient on the given rows (the training kernels,
+ * no update) */
+int mmf_learn_loss_grad(mmf_learn *net, const float *x, const float *t, const float *w, int64_t count,
+                        double *loss, float *grad);
+/* one Adam update (beta 0.9 / 0.999, eps 1e-8, bias-corrected) with a given gradient */
+int mmf_learn_adam_step(mmf_learn *net, const float *grad, double lr);
+/* 'steps' Adam steps on batches of 'batch' rows drawn from the training set,
+ * learning rate cosine-annealed from lr0 to lr1; final_loss = mean of the
+ * last 50 batch losses */
+int mmf_learn_train(mmf_learn *net, int32_t steps, int32_t batch, double lr0, double lr1, uint64_t seed,
+                    double *final_loss);
+/* raw network outputs z[count][16]: fp32 SIMT (use_mfma 0) or the bf16
+ * matrix-core path */
+int mmf_learn_forward(mmf_learn *net, const float *x, int64_t count, int32_t use_mfma, float *y);
+/* switch a 2D order-2 context without bodies to the learned face values
+ * (enable 1) or back (enable 0).  net NULL keeps the plain face values
+ * everywhere through the same face-state kernels.  The network must outlive
+ * the context's use of it. */
+int mmf_set_learned_reconstruction(mmf_ctx *ctx, const mmf_learn *net, double sensor_threshold, int32_t enable);
+/* optional safeguards on the learned face values, applied before the
+ * positivity check: clip (default 0) bounds each component between the means
+ * of the two cells sharing the face; blend (default 1) returns plain + blend
+ * (learned - plain) */
+int mmf_set_learned_options(mmf_ctx *ctx, int32_t clip, double blend);
+/* cells reconstructed by the network / on the fallback, summed over stage-1
+ * launches since the last reset */
+int mmf_get_learned_counts(mmf_ctx *ctx, int64_t *learned_cells, int64_t *fallback_cells);
+int mmf_reset_learned_counts(mmf_ctx *ctx);
+/* face states [ncells][4][5] (faces x-, x+, y-, y+; cells x-fastest) of the
+ * current state, as the stage-1 reconstruction produces them */
+int mmf_get_face_states(mmf_ctx *ctx, double *faces);
+/* conservative average of the current 2D state over factor x factor blocks,
+ * host AoS [ncells / factor^2][5] */
+int mmf_restrict_state(mmf_ctx *ctx, int32_t factor, double *coarse);
+
 /* ---- bandwidth calibration ------------------------------------------------ */
 /* Copies nbytes device-to-device with a plain vector-load/store kernel reps
  * times; returns the average kernel milliseconds (HIP events). */

@@ -82,6 +82,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     vp, dp, i64p, u8p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint8)
     i32p = C.POINTER(C.c_int32)
+    fp = C.POINTER(C.c_float)
     sig = {
         "mmf_create": ([C.POINTER(GridDesc), C.POINTER(C.c_void_p)], C.c_int),
         "mmf_destroy": ([vp], C.c_int),
@@ -138,6 +139,25 @@ def lib():
         "mmf_batch_enable_timing": ([vp, C.c_int32], C.c_int),
         "mmf_batch_read_timing": ([vp, dp, i64p], C.c_int),
         "mmf_batch_autotune": ([vp, C.c_int32, i32p, i32p, dp], C.c_int),
+        "mmf_learn_create": ([C.c_uint64, C.c_int64, C.POINTER(C.c_void_p)], C.c_int),
+        "mmf_learn_destroy": ([vp], C.c_int),
+        "mmf_learn_last_error": ([vp], C.c_char_p),
+        "mmf_learn_get_params": ([vp, fp], C.c_int),
+        "mmf_learn_set_params": ([vp, fp], C.c_int),
+        "mmf_learn_collect": ([vp, vp, C.c_int32, i64p, C.c_int32, C.c_double], C.c_int),
+        "mmf_learn_num_samples": ([vp, i64p], C.c_int),
+        "mmf_learn_get_samples": ([vp, C.c_int64, C.c_int64, fp, fp, fp], C.c_int),
+        "mmf_learn_set_samples": ([vp, fp, fp, fp, C.c_int64], C.c_int),
+        "mmf_learn_loss_grad": ([vp, fp, fp, fp, C.c_int64, dp, fp], C.c_int),
+        "mmf_learn_adam_step": ([vp, fp, C.c_double], C.c_int),
+        "mmf_learn_train": ([vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64, dp], C.c_int),
+        "mmf_learn_forward": ([vp, fp, C.c_int64, C.c_int32, fp], C.c_int),
+        "mmf_set_learned_reconstruction": ([vp, vp, C.c_double, C.c_int32], C.c_int),
+        "mmf_set_learned_options": ([vp, C.c_int32, C.c_double], C.c_int),
+        "mmf_get_learned_counts": ([vp, i64p, i64p], C.c_int),
+        "mmf_reset_learned_counts": ([vp], C.c_int),
+        "mmf_get_face_states": ([vp, dp], C.c_int),
+        "mmf_restrict_state": ([vp, C.c_int32, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -428,6 +448,36 @@ class Solver:
         self._c(lib().mmf_read_timing(self.ctx, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    # learned face reconstruction (2D, order 2)
+    def set_learned(self, net, sensor_threshold=0.05, enable=True):
+        """Use the network's face values (net None: plain face values through
+        the same face-state kernels); enable False restores the plain kernels."""
+        self._net = net
+        self._c(lib().mmf_set_learned_reconstruction(self.ctx, None if net is None else net.h,
+                                                     float(sensor_threshold), 1 if enable else 0))
+
+    def set_learned_options(self, clip=True, blend=1.0):
+        self._c(lib().mmf_set_learned_options(self.ctx, 1 if clip else 0, float(blend)))
+
+    def learned_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._c(lib().mmf_get_learned_counts(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def reset_learned_counts(self):
+        self._c(lib().mmf_reset_learned_counts(self.ctx))
+
+    def face_states(self):
+        out = np.empty((self.ncells, 4, NF))
+        self._c(lib().mmf_get_face_states(self.ctx, _dp(out)))
+        return out
+
+    def restrict_state(self, factor):
+        n = self.ncells // (int(factor) ** 2)
+        out = np.empty((n, NF))
+        self._c(lib().mmf_restrict_state(self.ctx, int(factor), _dp(out)))
+        return out
+
 
 # ---- batched mode: many independent 2D grids -------------------------------------
 
@@ -556,3 +606,99 @@ class Batch:
         self._c(lib().mmf_batch_autotune(self.h, int(steps_per_trial), C.byref(a), C.byref(b), C.byref(ms)))
         self.set_shape(0, 0)
         return a.value, b.value, ms.value
+
+
+# ---- learned face reconstruction -----------------------------------------------------
+
+LEARN_NPARAMS, LEARN_NIN, LEARN_NOUT = 11664, 100, 16
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Learner:
+    """The network of the learned face reconstruction, its Adam state and a
+    device-resident training set (include/mmf.h, mmf_learn_*)."""
+
+    def __init__(self, seed=0, capacity=0):
+        h = C.c_void_p()
+        _check(lib().mmf_learn_create(int(seed) & ((1 << 64) - 1), int(capacity), C.byref(h)))
+        self.h = h
+        self.capacity = int(capacity)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mmf_learn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        if rc != OK:
+            m = lib().mmf_learn_last_error(self.h) if getattr(self, "h", None) else None
+            raise MMFError(rc, m.decode() if m else "")
+
+    def params(self):
+        p = np.empty(LEARN_NPARAMS, dtype=np.float32)
+        self._c(lib().mmf_learn_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p, (LEARN_NPARAMS,))
+        self._c(lib().mmf_learn_set_params(self.h, _fp(p)))
+
+    def collect(self, fine, factor, cells, sensor_threshold=0.05):
+        c = np.ascontiguousarray(cells, dtype=np.int64)
+        self._c(lib().mmf_learn_collect(self.h, fine.ctx, int(factor), c.ctypes.data_as(C.POINTER(C.c_int64)),
+                                        c.size, float(sensor_threshold)))
+
+    def num_samples(self):
+        n = C.c_int64()
+        self._c(lib().mmf_learn_num_samples(self.h, C.byref(n)))
+        return n.value
+
+    def samples(self, first=0, count=None):
+        if count is None:
+            count = self.num_samples() - first
+        x = np.empty((count, LEARN_NIN), dtype=np.float32)
+        t = np.empty((count, LEARN_NOUT), dtype=np.float32)
+        w = np.empty(count, dtype=np.float32)
+        self._c(lib().mmf_learn_get_samples(self.h, int(first), int(count), _fp(x), _fp(t), _fp(w)))
+        return x, t, w
+
+    def set_samples(self, x, t, w):
+        x, t, w = _f32(x, (-1, LEARN_NIN)), _f32(t, (-1, LEARN_NOUT)), _f32(w, (-1,))
+        self._c(lib().mmf_learn_set_samples(self.h, _fp(x), _fp(t), _fp(w), x.shape[0]))
+
+    def loss_grad(self, x, t, w):
+        x, t, w = _f32(x, (-1, LEARN_NIN)), _f32(t, (-1, LEARN_NOUT)), _f32(w, (-1,))
+        loss = C.c_double()
+        g = np.empty(LEARN_NPARAMS, dtype=np.float32)
+        self._c(lib().mmf_learn_loss_grad(self.h, _fp(x), _fp(t), _fp(w), x.shape[0], C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def adam_step(self, grad, lr):
+        g = _f32(grad, (LEARN_NPARAMS,))
+        self._c(lib().mmf_learn_adam_step(self.h, _fp(g), float(lr)))
+
+    def train(self, steps, batch=8192, lr0=2e-3, lr1=2e-5, seed=1):
+        loss = C.c_double()
+        self._c(lib().mmf_learn_train(self.h, int(steps), int(batch), float(lr0), float(lr1),
+                                      int(seed) & ((1 << 64) - 1), C.byref(loss)))
+        return loss.value
+
+    def forward(self, x, mfma=True):
+        x = _f32(x, (-1, LEARN_NIN))
+        y = np.empty((x.shape[0], LEARN_NOUT), dtype=np.float32)
+        self._c(lib().mmf_learn_forward(self.h, _fp(x), x.shape[0], 1 if mfma else 0, _fp(y)))
+        return y

@@ -16,33 +16,10 @@
 #include <vector>
 
 #include "mmf.h"
+#include "mmf_ctx.hpp"
 #include "mmf_internal.hpp"
+#include "mmf_learn.hpp"
 
-using mmf::Ctrl;
-using mmf::GridDev;
-
-struct mmf_ctx {
-  mmf_grid_desc desc;
-  GridDev g;
-  int device = 0;
-  int num_cus = 0;
-  hipStream_t stream = nullptr;
-  double *dU = nullptr, *dR = nullptr, *dW = nullptr;
-  uint8_t *dflags = nullptr;
-  Ctrl *dctrl = nullptr;
-  Ctrl *hctrl = nullptr;  // pinned mirror for read-back
-  double *dpartials = nullptr;
-  int64_t nerr_blocks = 0;
-  double *derr = nullptr;
-  std::vector<int64_t> raw;  // cell_of_raw, empty when identity
-  int kernel = mmf::KERNEL_AUTO;
-  mmf::MarchShape shape{8, 32};
-  std::string err;
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-  double timed_ms = 0.0;
-  int64_t timed_launches = 0;
-};
 
 namespace {
 
@@ -99,11 +76,20 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
     HIPCHK(ctx, hipEventCreate(&e1));
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
   }
-  const bool march = ctx->kernel != mmf::KERNEL_GATHER && mmf::march_shape_supported(ctx->g, ctx->shape);
-  hipError_t e = march ? mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR,
-                                                 ctx->dW, ctx->dU, ctx->dctrl, ctx->stream)
-                       : mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
-                                           ctx->dctrl, ctx->stream);
+  hipError_t e;
+  if (ctx->learn_on) {
+    // face states (learned or plain), then flux + RK stage update from them
+    e = mmf::launch_recon_learned(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dctrl, ctx->stream);
+    if (e == hipSuccess)
+      e = mmf::launch_flux_fs(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
+                              ctx->dctrl, ctx->stream);
+  } else {
+    const bool march = ctx->kernel != mmf::KERNEL_GATHER && mmf::march_shape_supported(ctx->g, ctx->shape);
+    e = march ? mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW,
+                                        ctx->dU, ctx->dctrl, ctx->stream)
+              : mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
+                                  ctx->dctrl, ctx->stream);
+  }
   if (e != hipSuccess) return hip_fail(ctx, e, "stage kernel launch");
   if (ctx->timing) {
     HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
@@ -238,6 +224,8 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->hctrl) (void)hipHostFree(ctx->hctrl);
   if (ctx->dpartials) (void)hipFree(ctx->dpartials);
   if (ctx->derr) (void)hipFree(ctx->derr);
+  if (ctx->dFS) (void)hipFree(ctx->dFS);
+  if (ctx->dlcount) (void)hipFree(ctx->dlcount);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMF_OK;
@@ -279,6 +267,7 @@ int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *nifaces) {
 
 int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved) {
   if (!ctx) return MMF_ERR_INVALID;
+  if (ctx->learn_on) return fail(ctx, MMF_ERR_STATE, "the learned reconstruction runs on grids without bodies");
   const int64_t N = ctx->g.N;
   std::vector<uint8_t> f((size_t)N);
   bool all = true;
@@ -306,6 +295,7 @@ int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved
 
 int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nb, const double *spheres, int32_t ns) {
   if (!ctx || nb < 0 || ns < 0 || (nb && !boxes) || (ns && !spheres)) return fail(ctx, MMF_ERR_INVALID, "bodies");
+  if (ctx->learn_on) return fail(ctx, MMF_ERR_STATE, "the learned reconstruction runs on grids without bodies");
   const int64_t N = ctx->g.N;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   uint8_t *dfl = ctx->dflags;

@@ -1,0 +1,44 @@
+// mmf_ctx.hpp -- the device-resident state of one grid behind mmf_ctx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mmf.h"
+#include "mmf_internal.hpp"
+#include "mmf_learn.hpp"
+
+using mmf::Ctrl;
+using mmf::GridDev;
+
+struct mmf_ctx {
+  mmf_grid_desc desc;
+  GridDev g;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  double *dU = nullptr, *dR = nullptr, *dW = nullptr;
+  uint8_t *dflags = nullptr;
+  Ctrl *dctrl = nullptr;
+  Ctrl *hctrl = nullptr;  // pinned mirror for read-back
+  double *dpartials = nullptr;
+  int64_t nerr_blocks = 0;
+  double *derr = nullptr;
+  std::vector<int64_t> raw;  // cell_of_raw, empty when identity
+  int kernel = mmf::KERNEL_AUTO;
+  mmf::MarchShape shape{8, 32};
+  std::string err;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  double timed_ms = 0.0;
+  int64_t timed_launches = 0;
+  // learned face reconstruction (2D, order 2): active when learn_on
+  bool learn_on = false;
+  mmf::LearnRun learn{};
+  bool learn_opts_set = false;
+  double *dFS = nullptr;
+  unsigned long long *dlcount = nullptr;
+};

@@ -1,0 +1,859 @@
+// mmf_learn.hip -- learned face reconstruction on gfx950 (2D, order 2).
+//
+// The reference evaluates face values of its limited linear polynomial
+// (reconstruction.cpp:67-97: mean +/- slope/2, mean when the face density or
+// pressure is not positive).  The learned form replaces those face values by
+// the output of a small network evaluated on the matrix cores and keeps the
+// limited linear values as the fallback:
+//
+//   k_recon_learned  one block = 16x4 cells; the 20x8 region (2-cell halo) of
+//                    the stage state is staged in LDS; per cell the 5x5
+//                    stencil of rho, rho u, rho v, rho E is normalised
+//                    (centre c, scale s per component) and fed, 16 cells per
+//                    wave, through the 100-64-64-16 network with bf16
+//                    v_mfma_f32_16x16x16 and fp32 accumulation.  Output z of
+//                    face f, component v sets the face value P + sigmoid(z)
+//                    (Q - P) between the minmod value P and the fifth-order
+//                    upwind-biased linear value Q (a learned limiter).  The network
+//                    is applied transposed (H^T = W X^T): the accumulator of
+//                    one layer is, lane for lane, the B operand of the next,
+//                    so activations never leave registers.  A cell falls back
+//                    to the plain face values when its stencil crosses a
+//                    non-periodic border, when the shock sensor fires, or
+//                    when a learned face state has non-positive density or
+//                    pressure.  Face states go to a [face][component] SoA.
+//   k_flux_fs        one thread per cell: the four faces from the face states
+//                    (border states from the BC), Rusanov flux, residual in
+//                    the reference face order, fused RK stage update.  With
+//                    every cell on the fallback this is bit-identical to the
+//                    order-2 gather kernel.
+//
+// Training (fp32, deterministic): k_train evaluates forward and backward for
+// 32 rows per block with the weights staged in LDS and writes per-block
+// gradient partials; k_reduce sums them in block order; k_adam updates.
+// k_collect builds training rows from a fine run: coarse stencils by
+// conservative averaging, targets from the fine plain face values, eight
+// dihedral images of each row.
+#include <hip/hip_runtime.h>
+
+#include "mmf_device.hpp"
+#include "mmf_internal.hpp"
+#include "mmf_learn.hpp"
+#include "mmf_stage.hpp"
+
+namespace mmf {
+
+using namespace learn;
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// conservative component of stencil variable v (rho, rho u, rho v, rho E)
+__device__ __forceinline__ int var_of(int v) { return v == 3 ? FID_RHO_E : v; }
+
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+__device__ __forceinline__ float gelu_d(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// fp32 -> bf16, round to nearest even
+__device__ __forceinline__ short f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// 16 cells per wave through the network on the matrix cores.  Lane l holds
+// cell l%16 and input group g = l/16: xb[kb] = inputs 16 kb + 4 g + (0..3).
+// Returns y[i] = output 4 g + i (face g, variable i) of cell l%16.
+// B: biases b1 (64), b2 (64), b3 (16).
+__device__ __forceinline__ f4 mlp16(const s4 *F, const float *B, const s4 *xb, int lane) {
+  const int g = lane >> 4;
+  s4 h1[4], h2[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 7; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(mt * 7 + kb) * 64 + lane], xb[kb], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h1[mt][i] = f2bf(gelu_f(acc[i] + B[16 * mt + 4 * g + i]));
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(28 + mt * 4 + kb) * 64 + lane], h1[kb], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h2[mt][i] = f2bf(gelu_f(acc[i] + B[NH + 16 * mt + 4 * g + i]));
+  }
+  f4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(44 + kb) * 64 + lane], h2[kb], y, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) y[i] += B[2 * NH + 4 * g + i];
+  return y;
+}
+
+__device__ __forceinline__ float minmodf(float a, float b) {
+  if (a > 0.f && b > 0.f) return (a < b) ? a : b;
+  if (a < 0.f && b < 0.f) return (a > b) ? a : b;
+  return 0.f;
+}
+
+// Limiter candidates of output o from a normalised stencil row: P = minmod
+// face value, Q = fifth-order upwind-biased linear face value, along the
+// face's axis with offsets counted towards the face.
+__device__ __forceinline__ void candidates_norm(const float *x, int o, float &P, float &Q) {
+  const int f = o >> 2, v = o & 3, a = f >> 1, side = f & 1;
+  float u[5];
+#pragma unroll
+  for (int d = -2; d <= 2; ++d) {
+    const int dd = side ? d : -d;
+    u[d + 2] = a ? x[v * 25 + (dd + 2) * 5 + 2] : x[v * 25 + 2 * 5 + (dd + 2)];
+  }
+  P = u[2] + 0.5f * minmodf(u[3] - u[2], u[2] - u[1]);
+  Q = ((((2.0f * u[0] - 13.0f * u[1]) + 47.0f * u[2]) + 27.0f * u[3]) - 3.0f * u[4]) / 60.0f;
+}
+
+}  // namespace
+
+// ---- weight packing ------------------------------------------------------------
+// Fragment fi (layer 1: mt*7+kb; layer 2: 28+mt*4+kb; layer 3: 44+kb), lane l,
+// element i holds W[16 mt + l%16][16 kb + 4 (l/16) + i] (zero past K).
+__global__ __launch_bounds__(64) void k_learn_pack(const float *P, short *frags) {
+  const int fi = blockIdx.x, lane = threadIdx.x;
+  int off, K, mt, kb;
+  if (fi < 28) {
+    off = OFF_W1; K = NIN; mt = fi / 7; kb = fi % 7;
+  } else if (fi < 44) {
+    off = OFF_W2; K = NH; mt = (fi - 28) / 4; kb = (fi - 28) % 4;
+  } else {
+    off = OFF_W3; K = NH; mt = 0; kb = fi - 44;
+  }
+  const int m = 16 * mt + (lane & 15);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = 16 * kb + 4 * (lane >> 4) + i;
+    const float w = (k < K) ? P[off + m * K + k] : 0.0f;
+    frags[(fi * 64 + lane) * 4 + i] = f2bf(w);
+  }
+}
+
+hipError_t launch_learn_pack(const float *params, short *frags, hipStream_t s) {
+  hipLaunchKernelGGL(k_learn_pack, dim3(NFRAG), dim3(64), 0, s, params, frags);
+  return hipGetLastError();
+}
+
+// ---- inference on given rows (tests, offline use) -------------------------------
+__global__ __launch_bounds__(64) void k_learn_forward_mfma(const float *P, const s4 *F, const float *X, int64_t n,
+                                                           float *Y) {
+  __shared__ float sB[2 * NH + NOUT];
+  const int lane = threadIdx.x, c = lane & 15, g = lane >> 4;
+  for (int e = lane; e < 2 * NH + NOUT; e += 64)
+    sB[e] = (e < NH) ? P[OFF_B1 + e] : (e < 2 * NH) ? P[OFF_B2 + e - NH] : P[OFF_B3 + e - 2 * NH];
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * 16 + c;
+  const bool live = row < n;
+  s4 xb[7];
+#pragma unroll
+  for (int kb = 0; kb < 7; ++kb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = 16 * kb + 4 * g + i;
+      xb[kb][i] = f2bf((live && k < NIN) ? X[row * NIN + k] : 0.0f);
+    }
+  const f4 y = mlp16(F, sB, xb, lane);
+  if (live) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Y[row * NOUT + 4 * g + i] = y[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_learn_forward_f32(const float *P, const float *X, int64_t n, float *Y) {
+  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (row >= n) return;
+  const float *x = X + row * NIN;
+  float h1[NH], h2[NH];
+  for (int j = 0; j < NH; ++j) {
+    float z = 0.f;
+    for (int i = 0; i < NIN; ++i) z += P[OFF_W1 + j * NIN + i] * x[i];
+    h1[j] = gelu_f(z + P[OFF_B1 + j]);
+  }
+  for (int j = 0; j < NH; ++j) {
+    float z = 0.f;
+    for (int i = 0; i < NH; ++i) z += P[OFF_W2 + j * NH + i] * h1[i];
+    h2[j] = gelu_f(z + P[OFF_B2 + j]);
+  }
+  for (int o = 0; o < NOUT; ++o) {
+    float z = 0.f;
+    for (int i = 0; i < NH; ++i) z += P[OFF_W3 + o * NH + i] * h2[i];
+    Y[row * NOUT + o] = z + P[OFF_B3 + o];
+  }
+}
+
+hipError_t launch_learn_forward(const float *params, const short *frags, const float *X, int64_t n, int use_mfma,
+                                float *Y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (use_mfma)
+    hipLaunchKernelGGL(k_learn_forward_mfma, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, s, params,
+                       (const s4 *)frags, X, n, Y);
+  else
+    hipLaunchKernelGGL(k_learn_forward_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, params, X, n, Y);
+  return hipGetLastError();
+}
+
+// ---- training: forward + backward of TRAIN_ROWS rows per block ------------------
+// Rows: r = block * TRAIN_ROWS + t for r < rows; row r reads sample r, or with
+// nsamples > 0 a sample drawn by hashing (seed, r).  Loss sum w (y - t)^2 /
+// (16 rows); each block writes its gradient partial and its loss sum.
+constexpr int TR = TRAIN_ROWS;
+
+__global__ __launch_bounds__(256) void k_learn_train(const float *P, const float *X, const float *T, const float *Wt,
+                                                     int64_t rows, int64_t nsamples, uint64_t seed, float *partial,
+                                                     float *ploss) {
+  __shared__ float sW1[NH][NIN + 1];
+  __shared__ float sW2[NH][NH + 1];
+  __shared__ float sW3[NOUT][NH + 1];
+  __shared__ float sB[2 * NH + NOUT];
+  __shared__ float sx[TR][NIN];
+  __shared__ float sz1[TR][NH], sa1[TR][NH], sz2[TR][NH], sa2[TR][NH];
+  __shared__ float sdy[TR][NOUT], st[TR][NOUT], sw[TR];
+  __shared__ float sd2[TR][NH], sd1[TR][NH];
+  __shared__ float sred[256];
+  __shared__ int64_t ssrc[TR];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NH * NIN; e += 256) sW1[e / NIN][e % NIN] = P[OFF_W1 + e];
+  for (int e = tid; e < NH * NH; e += 256) sW2[e / NH][e % NH] = P[OFF_W2 + e];
+  for (int e = tid; e < NOUT * NH; e += 256) sW3[e / NH][e % NH] = P[OFF_W3 + e];
+  if (tid < 2 * NH + NOUT)
+    sB[tid] = (tid < NH) ? P[OFF_B1 + tid] : (tid < 2 * NH) ? P[OFF_B2 + tid - NH] : P[OFF_B3 + tid - 2 * NH];
+  if (tid < TR) {
+    const int64_t r = (int64_t)blockIdx.x * TR + tid;
+    int64_t src = -1;
+    if (r < rows)
+      src = (nsamples > 0) ? (int64_t)(mix64(seed + (uint64_t)r * 0x632BE59BD9B4E019ull) % (uint64_t)nsamples) : r;
+    ssrc[tid] = src;
+  }
+  __syncthreads();
+  for (int e = tid; e < TR * NIN; e += 256) {
+    const int s = e / NIN, i = e % NIN;
+    const int64_t src = ssrc[s];
+    sx[s][i] = (src >= 0) ? X[src * NIN + i] : 0.0f;
+  }
+  for (int e = tid; e < TR * NOUT; e += 256) {
+    const int s = e / NOUT, o = e % NOUT;
+    const int64_t src = ssrc[s];
+    st[s][o] = (src >= 0) ? T[src * NOUT + o] : 0.0f;
+  }
+  if (tid < TR) sw[tid] = (ssrc[tid] >= 0) ? Wt[ssrc[tid]] : 0.0f;
+  __syncthreads();
+
+  for (int e = tid; e < TR * NH; e += 256) {
+    const int s = e >> 6, j = e & 63;
+    float z = 0.f;
+    for (int i = 0; i < NIN; ++i) z += sW1[j][i] * sx[s][i];
+    z += sB[j];
+    sz1[s][j] = z;
+    sa1[s][j] = gelu_f(z);
+  }
+  __syncthreads();
+  for (int e = tid; e < TR * NH; e += 256) {
+    const int s = e >> 6, j = e & 63;
+    float z = 0.f;
+    for (int i = 0; i < NH; ++i) z += sW2[j][i] * sa1[s][i];
+    z += sB[NH + j];
+    sz2[s][j] = z;
+    sa2[s][j] = gelu_f(z);
+  }
+  __syncthreads();
+  const float scale = 2.0f / (float)(NOUT * rows);
+  float lsum = 0.f;
+  for (int e = tid; e < TR * NOUT; e += 256) {
+    const int s = e >> 4, o = e & 15;
+    float z = 0.f;
+    for (int i = 0; i < NH; ++i) z += sW3[o][i] * sa2[s][i];
+    z += sB[2 * NH + o];
+    // model output: P + sigmoid(z) (Q - P)
+    float Pn, Qn;
+    candidates_norm(&sx[s][0], o, Pn, Qn);
+    const float sg = 1.0f / (1.0f + expf(-z));
+    const float r = (Pn + sg * (Qn - Pn)) - st[s][o];
+    lsum += sw[s] * r * r;
+    sdy[s][o] = scale * sw[s] * r * (sg * (1.0f - sg)) * (Qn - Pn);
+  }
+  sred[tid] = lsum;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) sred[tid] += sred[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) ploss[blockIdx.x] = sred[0];
+
+  float *G = partial + (int64_t)blockIdx.x * NPARAMS;
+  for (int e = tid; e < NOUT * NH; e += 256) {
+    const int o = e >> 6, i = e & 63;
+    float a = 0.f;
+    for (int s = 0; s < TR; ++s) a += sdy[s][o] * sa2[s][i];
+    G[OFF_W3 + e] = a;
+  }
+  if (tid < NOUT) {
+    float a = 0.f;
+    for (int s = 0; s < TR; ++s) a += sdy[s][tid];
+    G[OFF_B3 + tid] = a;
+  }
+  for (int e = tid; e < TR * NH; e += 256) {
+    const int s = e >> 6, i = e & 63;
+    float a = 0.f;
+    for (int o = 0; o < NOUT; ++o) a += sW3[o][i] * sdy[s][o];
+    sd2[s][i] = a * gelu_d(sz2[s][i]);
+  }
+  __syncthreads();
+  for (int e = tid; e < NH * NH; e += 256) {
+    const int j = e >> 6, i = e & 63;
+    float a = 0.f;
+    for (int s = 0; s < TR; ++s) a += sd2[s][j] * sa1[s][i];
+    G[OFF_W2 + e] = a;
+  }
+  if (tid < NH) {
+    float a = 0.f;
+    for (int s = 0; s < TR; ++s) a += sd2[s][tid];
+    G[OFF_B2 + tid] = a;
+  }
+  for (int e = tid; e < TR * NH; e += 256) {
+    const int s = e >> 6, i = e & 63;
+    float a = 0.f;
+    for (int j = 0; j < NH; ++j) a += sW2[j][i] * sd2[s][j];
+    sd1[s][i] = a * gelu_d(sz1[s][i]);
+  }
+  __syncthreads();
+  for (int e = tid; e < NH * NIN; e += 256) {
+    const int j = e / NIN, i = e % NIN;
+    float a = 0.f;
+    for (int s = 0; s < TR; ++s) a += sd1[s][j] * sx[s][i];
+    G[OFF_W1 + e] = a;
+  }
+  if (tid < NH) {
+    float a = 0.f;
+    for (int s = 0; s < TR; ++s) a += sd1[s][tid];
+    G[OFF_B1 + tid] = a;
+  }
+}
+
+int64_t learn_train_blocks(int64_t rows) { return (rows + TR - 1) / TR; }
+
+hipError_t launch_learn_train(const float *params, const float *X, const float *T, const float *W, int64_t rows,
+                              int64_t nsamples, uint64_t seed, float *partial, float *ploss, hipStream_t s) {
+  hipLaunchKernelGGL(k_learn_train, dim3((unsigned)learn_train_blocks(rows)), dim3(256), 0, s, params, X, T, W, rows,
+                     nsamples, seed, partial, ploss);
+  return hipGetLastError();
+}
+
+// gradient = sum of the block partials in block order; loss = sum of the
+// block loss sums / (16 rows), in block order, in fp64
+__global__ __launch_bounds__(256) void k_learn_reduce(const float *partial, const float *ploss, int64_t nblocks,
+                                                      int64_t rows, float *grad, double *loss) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < NPARAMS) {
+    float a = 0.f;
+    for (int64_t b = 0; b < nblocks; ++b) a += partial[b * NPARAMS + p];
+    grad[p] = a;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && loss) {
+    double l = 0.0;
+    for (int64_t b = 0; b < nblocks; ++b) l += (double)ploss[b];
+    *loss = l / (double)(NOUT * rows);
+  }
+}
+
+hipError_t launch_learn_reduce(const float *partial, const float *ploss, int64_t nblocks, int64_t rows, float *grad,
+                               double *loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_learn_reduce, dim3((NPARAMS + 255) / 256), dim3(256), 0, s, partial, ploss, nblocks, rows, grad,
+                     loss);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_learn_adam(float *P, const float *G, float *M, float *V, float lr, float bc1,
+                                                    float bc2) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= NPARAMS) return;
+  const float g = G[p];
+  const float m = 0.9f * M[p] + 0.1f * g;
+  const float v = 0.999f * V[p] + 0.001f * (g * g);
+  M[p] = m;
+  V[p] = v;
+  P[p] = P[p] - lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f);
+}
+
+hipError_t launch_learn_adam(float *params, const float *grad, float *m, float *v, float lr, float bc1, float bc2,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_learn_adam, dim3((NPARAMS + 255) / 256), dim3(256), 0, s, params, grad, m, v, lr, bc1, bc2);
+  return hipGetLastError();
+}
+
+// ---- training rows from a fine run ------------------------------------------------
+// One block per selected coarse cell of the grid restricted by 'factor'.
+__global__ __launch_bounds__(256) void k_learn_collect(GridDev fg, const double *Uf, int r, const int64_t *cells,
+                                                       double theta, float *X, float *T, float *W) {
+  __shared__ double S[NF][5][5];
+  __shared__ uint8_t sex[5][5];
+  __shared__ double FT[4][8][NF];
+  __shared__ double cs[4][2];
+  __shared__ float xs[NIN], ts[NOUT];
+  __shared__ float wsh;
+  const int tid = threadIdx.x;
+  const int64_t fs = fg.fs, nfx = fg.n[0];
+  const int64_t ncx = fg.n[0] / r, ncy = fg.n[1] / r;
+  const int64_t cell = cells[blockIdx.x];
+  const int64_t I = cell % ncx, J = cell / ncx;
+  if (tid < NF * 25) {
+    const int k = tid / 25, rr = tid % 25, dj = rr / 5 - 2, di = rr % 5 - 2;
+    int64_t ci = I + di, cj = J + dj;
+    const bool ex = ((ci >= 0 && ci < ncx) || fg.periodic[0]) && ((cj >= 0 && cj < ncy) || fg.periodic[1]);
+    ci = ((ci % ncx) + ncx) % ncx;
+    cj = ((cj % ncy) + ncy) % ncy;
+    double acc = 0.0;
+    for (int b = 0; b < r; ++b)
+      for (int a = 0; a < r; ++a) acc += Uf[k * fs + (r * cj + b) * nfx + (r * ci + a)];
+    S[k][dj + 2][di + 2] = acc * (1.0 / (double)(r * r));
+    if (k == 0) sex[dj + 2][di + 2] = ex;
+  }
+  if (tid >= 128 && tid < 128 + 4 * r) {
+    const int q = tid - 128, f = q / r, b = q % r;
+    const int a = f >> 1, side = f & 1;
+    int64_t fq[3];
+    if (a == 0) {
+      fq[0] = r * I + (side ? r - 1 : 0);
+      fq[1] = r * J + b;
+    } else {
+      fq[0] = r * I + b;
+      fq[1] = r * J + (side ? r - 1 : 0);
+    }
+    fq[2] = 0;
+    const int64_t idx = fq[0] + nfx * fq[1];
+    int64_t nm = 0, np = 0;
+    const bool em = neighbour(fg, fq, a, -1, nm);
+    const bool ep = neighbour(fg, fq, a, +1, np);
+    double X0[NF], v[NF], pr[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      X0[k] = Uf[k * fs + idx];
+      const double dm = em ? X0[k] - Uf[k * fs + nm] : 0.0;
+      const double dp = ep ? Uf[k * fs + np] - X0[k] : 0.0;
+      const double sl = minmod(dp, dm);
+      v[k] = side ? X0[k] + 0.5 * sl : X0[k] - 0.5 * sl;
+    }
+    cons2prim(v, pr);
+    if (!(v[FID_RHO] > 0.0) || !(pr[FID_P] > 0.0)) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) v[k] = X0[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) FT[f][b][k] = v[k];
+  }
+  __syncthreads();
+  if (tid < 4) {
+    const int k = var_of(tid);
+    const double c = S[k][2][2];
+    double dev = 0.0;
+    for (int e = 0; e < 25; ++e) {
+      const double d = fabs(S[k][e / 5][e % 5] - c);
+      dev = (d > dev) ? d : dev;
+    }
+    const double tau = TAU_REL * (fabs(S[FID_RHO][2][2]) + fabs(S[FID_RHO_E][2][2]));
+    cs[tid][0] = c;
+    cs[tid][1] = (dev >= tau) ? dev : tau;
+  }
+  if (tid == 4) {
+    bool all = true;
+    for (int e = 0; e < 25; ++e) all = all && sex[e / 5][e % 5];
+    double phi = 0.0;
+    for (int a = 0; a < 2; ++a) {
+      const int mj = a ? 1 : 2, mi = a ? 2 : 1, pj = a ? 3 : 2, pi = a ? 2 : 3;
+      for (int qn = 0; qn < 2; ++qn) {
+        double q[3];
+        const int jj[3] = {mj, 2, pj}, ii[3] = {mi, 2, pi};
+        for (int t = 0; t < 3; ++t) {
+          if (qn == 0) {
+            q[t] = S[FID_RHO][jj[t]][ii[t]];
+          } else {
+            double c5[NF], p5[NF];
+            for (int k = 0; k < NF; ++k) c5[k] = S[k][jj[t]][ii[t]];
+            cons2prim(c5, p5);
+            q[t] = p5[FID_P];
+          }
+        }
+        const double val = fabs((q[2] - 2.0 * q[1]) + q[0]) / ((q[2] + 2.0 * q[1]) + q[0]);
+        phi = (val > phi) ? val : phi;
+      }
+    }
+    wsh = (all && phi <= theta) ? 1.0f : 0.0f;
+  }
+  __syncthreads();
+  if (tid < NIN) {
+    const int v = tid / 25, rr = tid % 25;
+    xs[tid] = (float)((S[var_of(v)][rr / 5][rr % 5] - cs[v][0]) / cs[v][1]);
+  }
+  if (tid >= 128 && tid < 128 + NOUT) {
+    const int o = tid - 128, f = o >> 2, v = o & 3;
+    double acc = 0.0;
+    for (int b = 0; b < r; ++b) acc += FT[f][b][var_of(v)];
+    ts[o] = (float)((acc * (1.0 / (double)r) - cs[v][0]) / cs[v][1]);
+  }
+  __syncthreads();
+  // eight dihedral images: transpose (bit 0), then reflect x (bit 1), then
+  // reflect y (bit 2); the source of each destination element is found by
+  // undoing the steps in reverse order
+  const int64_t base = (int64_t)blockIdx.x * 8;
+  for (int e = tid; e < 8 * NIN; e += 256) {
+    const int sg = e / NIN, d = e % NIN;
+    int v = d / 25, dj = (d % 25) / 5, di = d % 5;
+    float sgn = 1.0f;
+    if (sg & 4) { if (v == 2) sgn = -sgn; dj = 4 - dj; }
+    if (sg & 2) { if (v == 1) sgn = -sgn; di = 4 - di; }
+    if (sg & 1) { v = (v == 1) ? 2 : (v == 2) ? 1 : v; const int t = dj; dj = di; di = t; }
+    X[(base + sg) * NIN + d] = sgn * xs[v * 25 + dj * 5 + di];
+  }
+  for (int e = tid; e < 8 * NOUT; e += 256) {
+    const int sg = e / NOUT, d = e % NOUT;
+    int f = d >> 2, v = d & 3;
+    float sgn = 1.0f;
+    if (sg & 4) { if (v == 2) sgn = -sgn; f = (f == 2) ? 3 : (f == 3) ? 2 : f; }
+    if (sg & 2) { if (v == 1) sgn = -sgn; f = (f == 0) ? 1 : (f == 1) ? 0 : f; }
+    if (sg & 1) { f = (f + 2) & 3; v = (v == 1) ? 2 : (v == 2) ? 1 : v; }
+    T[(base + sg) * NOUT + d] = sgn * ts[f * 4 + v];
+  }
+  if (tid < 8) W[base + tid] = wsh;
+}
+
+hipError_t launch_learn_collect(const GridDev &fine, const double *Uf, int factor, const int64_t *cells, int ncells,
+                                double theta, float *X, float *T, float *W, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_learn_collect, dim3((unsigned)ncells), dim3(256), 0, s, fine, Uf, factor, cells, theta, X, T, W);
+  return hipGetLastError();
+}
+
+// conservative average over factor x factor blocks -> AoS [ncoarse][5]
+__global__ __launch_bounds__(256) void k_learn_restrict(GridDev fg, const double *Uf, int r, double *out) {
+  const int64_t ncx = fg.n[0] / r, ncy = fg.n[1] / r;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ncx * ncy * NF) return;
+  const int64_t cell = e / NF;
+  const int k = (int)(e % NF);
+  const int64_t I = cell % ncx, J = cell / ncx;
+  double acc = 0.0;
+  for (int b = 0; b < r; ++b)
+    for (int a = 0; a < r; ++a) acc += Uf[k * fg.fs + (r * J + b) * fg.n[0] + (r * I + a)];
+  out[e] = acc * (1.0 / (double)(r * r));
+}
+
+hipError_t launch_learn_restrict(const GridDev &fine, const double *Uf, int factor, double *coarse, hipStream_t s) {
+  const int64_t n = (fine.n[0] / factor) * (fine.n[1] / factor) * NF;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_learn_restrict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fine, Uf, factor, coarse);
+  return hipGetLastError();
+}
+
+// ---- reconstruction kernel ----------------------------------------------------------
+constexpr int RTX = 16, RTY = 4, RGX = RTX + 4, RGY = RTY + 4;
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, const double *U, const double *R,
+                                                       const double *Xin, Ctrl *ctrl, int mode) {
+  __shared__ double reg[NF][RGY][RGX];
+  __shared__ uint8_t rex[RGY][RGX];
+  __shared__ s4 sF[NFRAG * 64];
+  __shared__ float sB[2 * NH + NOUT];
+  __shared__ double scs[RTY][16][4][2];
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
+  const double dt = (STAGE == 2) ? step_dt(ctrl) : 0.0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
+  const int nx = (int)g.n[0], ny = (int)g.n[1];
+  const int64_t fs = g.fs;
+  const int x0 = blockIdx.x * RTX, y0 = blockIdx.y * RTY;
+  const bool use_net = run.net.frags != nullptr;
+
+  for (int e = tid; e < RGX * RGY; e += 256) {
+    const int ry = e / RGX, rx = e % RGX;
+    int gx = x0 - 2 + rx, gy = y0 - 2 + ry;
+    const bool ex = ((gx >= 0 && gx < nx) || g.periodic[0]) && ((gy >= 0 && gy < ny) || g.periodic[1]);
+    gx = ((gx % nx) + nx) % nx;
+    gy = ((gy % ny) + ny) % ny;
+    rex[ry][rx] = ex;
+    const int64_t i = gx + (int64_t)nx * gy;
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      reg[k][ry][rx] = !ex ? 0.0 : (STAGE == 2) ? U[k * fs + i] + div_vol(g, dt * R[k * fs + i]) : Xin[k * fs + i];
+  }
+  if (use_net) {
+    const s4 *gF = (const s4 *)run.net.frags;
+    for (int e = tid; e < NFRAG * 64; e += 256) sF[e] = gF[e];
+    const float *P = run.net.params;
+    if (tid < 2 * NH + NOUT)
+      sB[tid] = (tid < NH) ? P[OFF_B1 + tid] : (tid < 2 * NH) ? P[OFF_B2 + tid - NH] : P[OFF_B3 + tid - 2 * NH];
+  }
+  __syncthreads();
+
+  const int gx = x0 + c, gy = y0 + w;
+  const bool wave_live = gy < ny;
+  const bool live = wave_live && gx < nx;
+  const int rx = c + 2, ry = w + 2;
+  const bool learnable = live && (g.periodic[0] || (gx >= 2 && gx <= nx - 3)) &&
+                         (g.periodic[1] || (gy >= 2 && gy <= ny - 3));
+  bool fires = false;
+  if (use_net) {
+    // centre and scale of stencil variable grp of this lane's cell
+    const int kv = var_of(grp);
+    const double cv = reg[kv][ry][rx];
+    double dev = 0.0;
+    for (int dj = -2; dj <= 2; ++dj)
+      for (int di = -2; di <= 2; ++di) {
+        const double d = fabs(reg[kv][ry + dj][rx + di] - cv);
+        dev = (d > dev) ? d : dev;
+      }
+    const double tau = TAU_REL * (fabs(reg[FID_RHO][ry][rx]) + fabs(reg[FID_RHO_E][ry][rx]));
+    scs[w][c][grp][0] = cv;
+    scs[w][c][grp][1] = (dev >= tau) ? dev : tau;
+    // shock sensor: lane group 0 on rho, group 1 on p
+    if (grp < 2) {
+      double phi = 0.0;
+      for (int a = 0; a < 2; ++a) {
+        double q[3];
+        for (int t = 0; t < 3; ++t) {
+          const int yy = ry + (a ? t - 1 : 0), xx = rx + (a ? 0 : t - 1);
+          if (grp == 0) {
+            q[t] = reg[FID_RHO][yy][xx];
+          } else {
+            double c5[NF], p5[NF];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) c5[k] = reg[k][yy][xx];
+            cons2prim(c5, p5);
+            q[t] = p5[FID_P];
+          }
+        }
+        const double val = fabs((q[2] - 2.0 * q[1]) + q[0]) / ((q[2] + 2.0 * q[1]) + q[0]);
+        phi = (val > phi) ? val : phi;
+      }
+      fires = !(phi <= run.theta);
+    }
+  }
+  __syncthreads();
+
+  // plain limited linear face value of face grp (reconstruction.cpp:67-97)
+  double P[NF];
+  {
+    const int a = grp >> 1, side = grp & 1;
+    const int my = ry - a, mx = rx - (1 - a), py = ry + a, px = rx + (1 - a);
+    const bool em = rex[my][mx], ep = rex[py][px];
+    double X0[NF], pr[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      X0[k] = reg[k][ry][rx];
+      const double dm = em ? X0[k] - reg[k][my][mx] : 0.0;
+      const double dp = ep ? reg[k][py][px] - X0[k] : 0.0;
+      const double sl = minmod(dp, dm);
+      P[k] = side ? X0[k] + 0.5 * sl : X0[k] - 0.5 * sl;
+    }
+    cons2prim(P, pr);
+    if (!(P[FID_RHO] > 0.0) || !(pr[FID_P] > 0.0)) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) P[k] = X0[k];
+    }
+  }
+
+  double Fv[NF];
+  bool pos = false;
+  if (use_net && wave_live) {
+    s4 xb[7];
+#pragma unroll
+    for (int kb = 0; kb < 7; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = 16 * kb + 4 * grp + i;
+        float xv = 0.0f;
+        if (live && k < NIN) {
+          const int v = k / 25, rr = k % 25;
+          const double S = reg[var_of(v)][ry - 2 + rr / 5][rx - 2 + rr % 5];
+          xv = (float)((S - scs[w][c][v][0]) / scs[w][c][v][1]);
+        }
+        xb[kb][i] = f2bf(xv);
+      }
+    const f4 y = mlp16(sF, sB, xb, lane);
+    // face neighbour: the cell across face grp
+    const int a = grp >> 1, side = grp & 1;
+    const int ny_ = ry + (a ? (side ? 1 : -1) : 0), nx_ = rx + (a ? 0 : (side ? 1 : -1));
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int k = var_of(v);
+      // learned limiter: between the minmod value and the fifth-order
+      // upwind-biased value, offsets counted towards the face
+      double u[5];
+#pragma unroll
+      for (int d = -2; d <= 2; ++d) {
+        const int dd = side ? d : -d;
+        u[d + 2] = a ? reg[k][ry + dd][rx] : reg[k][ry][rx + dd];
+      }
+      const double Pm = u[2] + 0.5 * minmod(u[3] - u[2], u[2] - u[1]);
+      const double Q5 = ((((2.0 * u[0] - 13.0 * u[1]) + 47.0 * u[2]) + 27.0 * u[3]) - 3.0 * u[4]) / 60.0;
+      const double sg = (double)(1.0f / (1.0f + expf(-y[v])));
+      double f = Pm + sg * (Q5 - Pm);
+      if (run.clip) {
+        // between the means of the two cells sharing the face
+        const double u0 = reg[k][ry][rx], u1 = reg[k][ny_][nx_];
+        const double lo = (u0 < u1) ? u0 : u1, hi = (u0 < u1) ? u1 : u0;
+        f = (f < lo) ? lo : (f > hi) ? hi : f;
+      }
+      if (run.blend != 1.0) f = P[k] + run.blend * (f - P[k]);
+      Fv[k] = f;
+    }
+    Fv[FID_RHO_W] = reg[FID_RHO_W][ry][rx];
+    double pr[NF];
+    cons2prim(Fv, pr);
+    pos = (Fv[FID_RHO] > 0.0) && (pr[FID_P] > 0.0);
+  }
+  const unsigned long long fire_m = __ballot(fires);
+  const unsigned long long bad_m = __ballot(!pos);
+  const bool cell_fire = ((fire_m >> c) | (fire_m >> (c + 16))) & 1ull;
+  const bool cell_bad = ((bad_m >> c) | (bad_m >> (c + 16)) | (bad_m >> (c + 32)) | (bad_m >> (c + 48))) & 1ull;
+  const bool learned = use_net && learnable && !cell_fire && !cell_bad;
+
+  if (live && !learned) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) Fv[k] = P[k];
+  }
+  if (live) {
+    const int64_t cell = gx + (int64_t)nx * gy;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) run.fs[(grp * NF + k) * fs + cell] = Fv[k];
+  }
+  const unsigned long long lm = __ballot(live && learned && grp == 0);
+  const unsigned long long fm = __ballot(live && !learned && grp == 0);
+  if (lane == 0 && STAGE == 1) {
+    if (lm) atomicAdd(&run.count[0], (unsigned long long)__popcll(lm));
+    if (fm) atomicAdd(&run.count[1], (unsigned long long)__popcll(fm));
+  }
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_flux_fs(GridDev g, StageIn<STAGE> in, const double *FS, const double *U,
+                                                 double *out, Ctrl *ctrl, int mode) {
+  const int slot = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;
+  if (STAGE >= 2) in.dt = step_dt(ctrl);
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double lam_max = 0.0;
+  if (idx < g.N) {
+    const int64_t fs = g.fs;
+    const int64_t q[3] = {idx % g.n[0], idx / g.n[0], 0};
+    double Xc[NF];
+    in.load(g, idx, Xc);
+    double rhs[NF] = {0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      int64_t nm1 = 0, np1 = 0;
+      const bool em1 = neighbour(g, q, a, -1, nm1);
+      const bool ep1 = neighbour(g, q, a, +1, np1);
+      double Lm[NF], Rm[NF], Lp[NF], Rp[NF];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        Rm[k] = FS[((2 * a) * NF + k) * fs + idx];
+        Lp[k] = FS[((2 * a + 1) * NF + k) * fs + idx];
+      }
+      if (em1) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Lm[k] = FS[((2 * a + 1) * NF + k) * fs + nm1];
+      } else {
+        border_state(g, 2 * a, a, Rm, Lm);
+      }
+      if (ep1) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Rp[k] = FS[((2 * a) * NF + k) * fs + np1];
+      } else {
+        border_state(g, 2 * a + 1, a, Lp, Rp);
+      }
+      Tup TLm, TRm, TLp, TRp;
+      make_tup(Rm, TRm);
+      make_tup(Lp, TLp);
+      make_tup(Lm, TLm);
+      make_tup(Rp, TRp);
+      double Fm[NF], Fp[NF];
+      const double lm = face_flux(TLm, TRm, a, Fm);
+      const double lp = face_flux(TLp, TRp, a, Fp);
+      lam_max = (lm < lam_max) ? lam_max : lm;
+      lam_max = (lp < lam_max) ? lam_max : lp;
+      if (!em1) count_neg(TLm, TRm, ctrl);
+      count_neg(TLp, TRp, ctrl);
+      const double A = g.A;
+      const bool wrap_first = g.periodic[a] && (q[a] == g.n[a] - 1);
+      if (wrap_first) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) { rhs[k] -= A * Fp[k]; rhs[k] += A * Fm[k]; }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) { rhs[k] += A * Fm[k]; rhs[k] -= A * Fp[k]; }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const double u = (STAGE == 1) ? 0.0 : U[k * fs + idx];
+      out[k * fs + idx] = rk_update<STAGE>(g, u, Xc[k], rhs[k], in.dt);
+    }
+  }
+  block_max_to<256>(&ctrl->eig[slot], lam_max, threadIdx.x);
+}
+
+template <int STAGE>
+static hipError_t recon_t(const GridDev &g, const LearnRun &run, int mode, const double *U, const double *R,
+                          const double *W, Ctrl *ctrl, hipStream_t s) {
+  const double *X = (STAGE == 3) ? W : U;
+  dim3 grid((unsigned)((g.n[0] + RTX - 1) / RTX), (unsigned)((g.n[1] + RTY - 1) / RTY));
+  hipLaunchKernelGGL((k_recon_learned<STAGE>), grid, dim3(256), 0, s, g, run, U, R, X, ctrl, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_recon_learned(const GridDev &g, const LearnRun &run, int stage, int mode, const double *U,
+                                const double *R, const double *W, Ctrl *ctrl, hipStream_t s) {
+  switch (stage) {
+    case 1: return recon_t<1>(g, run, mode, U, R, W, ctrl, s);
+    case 2: return recon_t<2>(g, run, mode, U, R, W, ctrl, s);
+    default: return recon_t<3>(g, run, mode, U, R, W, ctrl, s);
+  }
+}
+
+template <int STAGE>
+static hipError_t flux_t(const GridDev &g, const LearnRun &run, int mode, const double *U, const double *R,
+                         const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  StageIn<STAGE> in;
+  in.fs = g.fs;
+  in.dt = 0.0;
+  in.U = U;
+  in.R = R;
+  in.X = (STAGE == 3) ? W : U;
+  double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
+  hipLaunchKernelGGL((k_flux_fs<STAGE>), dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, in, run.fs, U, out,
+                     ctrl, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_fs(const GridDev &g, const LearnRun &run, int stage, int mode, const double *U, const double *R,
+                          const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  switch (stage) {
+    case 1: return flux_t<1>(g, run, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return flux_t<2>(g, run, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return flux_t<3>(g, run, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
+}
+
+}  // namespace mmf

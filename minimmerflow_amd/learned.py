@@ -1,0 +1,121 @@
+"""Training and evaluation of the learned face reconstruction (2D, order 2).
+
+Data come from the solver itself: fine plain runs (order 2, limited linear)
+of seeded cases, snapshotted during the run; each snapshot contributes the
+rows of randomly drawn coarse cells of the state restricted by the factor
+(mmf_learn_collect).  Cases:
+
+  vortex  isentropic vortex (problem.cpp:247-318) on the periodic box
+          [-5,5]^2, beta ~ U[3,6], centre ~ U[-2,2]^2, free stream u = v = 1;
+          error against the exact (periodically wrapped) solution;
+  sod     radial Sod (problem.cpp:319-355) on [-1,1]^2, r0 = 0.4, free-flow
+          borders, inner rho = p = 1, density and pressure ratios ~ U[4,16];
+          error against the fine plain run restricted to the coarse grid.
+
+Everything here drives the C-ABI; the arithmetic runs in the HIP kernels.
+"""
+import time
+
+import numpy as np
+
+from . import BC_FREE_FLOW, BC_PERIODIC, Learner, Solver, problem
+from .workloads import SplitMix64
+
+VORTEX_TEND = 2.0
+SOD_TEND = 0.2
+CFL = 0.5
+
+
+def vortex_problem(beta, center):
+    return problem("vortex_xy", 2, vortex_beta=beta, vortex_center=[center[0], center[1], 0.0], vortex_periodic=1)
+
+
+def sod_problem(density_ratio, pressure_ratio):
+    return problem("radsod", 2, domain_origin=[-1.0, -1.0, 0.0], domain_length=2.0, sod_r0=0.4,
+                   sod_p_in=1.0, sod_t_in=1.0, sod_p_out=1.0 / pressure_ratio,
+                   sod_t_out=density_ratio / pressure_ratio)
+
+
+def case_params(n, seed, kinds=("vortex", "sod")):
+    """n seeded cases alternating over 'kinds'."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(n):
+        kind = kinds[i % len(kinds)]
+        if kind == "vortex":
+            out.append({"kind": "vortex", "beta": rng.uniform(3.0, 6.0),
+                        "center": (rng.uniform(-2.0, 2.0), rng.uniform(-2.0, 2.0))})
+        else:
+            out.append({"kind": "sod", "density_ratio": rng.uniform(4.0, 16.0),
+                        "pressure_ratio": rng.uniform(4.0, 16.0)})
+    return out
+
+
+def make_problem(c):
+    if c["kind"] == "vortex":
+        return vortex_problem(c["beta"], c["center"])
+    return sod_problem(c["density_ratio"], c["pressure_ratio"])
+
+
+def end_time(c):
+    return VORTEX_TEND if c["kind"] == "vortex" else SOD_TEND
+
+
+def side_bc(c):
+    return [BC_PERIODIC] * 4 if c["kind"] == "vortex" else [BC_FREE_FLOW] * 4
+
+
+def solver(c, n):
+    pb = make_problem(c)
+    s = Solver.for_problem(pb, n, order=2, side_bc=side_bc(c))
+    s.set_initial_state(pb)
+    return s, pb
+
+
+def collect_case(net, c, n_fine, factor, snapshots, cells_per_snapshot, seed, theta=0.05):
+    """One fine plain run of case c to its end time, with 'snapshots' evenly
+    spaced snapshots of 'cells_per_snapshot' random coarse cells each."""
+    s, _ = solver(c, n_fine)
+    tend = end_time(c)
+    nc = (n_fine // factor) ** 2
+    rng = np.random.default_rng(seed)
+    s.set_time(0.0, tend, CFL)
+    for k in range(snapshots):
+        s.set_time(s.time()[0], tend * (k + 1) / snapshots, CFL)
+        s.run()
+        net.collect(s, factor, rng.integers(0, nc, cells_per_snapshot), theta)
+    s.close()
+
+
+def build_training_set(net, cases, n_fine, factor, snapshots, cells_per_snapshot, seed=1, theta=0.05, log=None):
+    t0 = time.perf_counter()
+    for i, c in enumerate(cases):
+        collect_case(net, c, n_fine, factor, snapshots, cells_per_snapshot, seed * 1000003 + i, theta)
+        if log and (i + 1) % max(1, len(cases) // 4) == 0:
+            log("collected %d/%d cases, %d rows, %.1fs" % (i + 1, len(cases), net.num_samples(),
+                                                            time.perf_counter() - t0))
+    return net.num_samples()
+
+
+def run_case(c, n, net=None, theta=0.05):
+    """Run case c at n^2 to its end time (plain, or learned when net is given).
+    Returns (solver, steps, wall seconds of the stepping loop)."""
+    s, pb = solver(c, n)
+    if net is not None:
+        s.set_learned(net, theta)
+    s.set_time(0.0, end_time(c), CFL)
+    s.synchronize()
+    t0 = time.perf_counter()
+    steps, _ = s.run()
+    s.synchronize()
+    return s, pb, steps, time.perf_counter() - t0
+
+
+def l1_error(c, s, pb, reference=None, factor=1):
+    """L1 density error (main.cpp:553-561): against the exact solution for a
+    vortex, against 'reference' (a fine solver, restricted by 'factor') for Sod."""
+    if c["kind"] == "vortex":
+        return s.error_vs_exact(pb, end_time(c))[0]
+    rho_ref = reference.restrict_state(factor)[:, 0]
+    l1, _ = s.error_norms(rho_ref)
+    return l1

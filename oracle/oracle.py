@@ -114,6 +114,7 @@ def _load(path):
     L.orc_run.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int64, dp]
     L.orc_run.restype = C.c_int64
     L.orc_error.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
+    L.orc_reconstruct.argtypes = [C.c_void_p, C.c_int64, dp, dp, dp]
     L.orc_mark_fluid.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int, C.POINTER(C.c_uint8)]
     _lib = L
     return L
@@ -257,6 +258,23 @@ class Solver:
         lib().orc_compute_polynomials(self.s, _dp(U))
         lib().orc_compute_rhs(self.s, _dp(U), _dp(R), C.byref(me))
         return R, me.value
+
+    def face_states(self, U):
+        """Order-2 face values of every cell, faces x-, x+, y-, y+ (, z-, z+):
+        orc_compute_polynomials + orc_reconstruct per face."""
+        U = _f64(U)
+        L = lib()
+        L.orc_compute_polynomials(self.s, _dp(U))
+        nfc = 2 * self.dim
+        out = np.empty((self.ncells, nfc, NF))
+        v = np.empty(NF)
+        for c in range(self.ncells):
+            for f in range(nfc):
+                n = np.zeros(3)
+                n[f // 2] = -1.0 if f % 2 == 0 else 1.0
+                L.orc_reconstruct(self.s, c, _dp(n), _dp(U), _dp(v))
+                out[c, f] = v
+        return out
 
     def rk3_step(self, U, W, cfl, t, tmax):
         """In-place step of U (and work W); returns (dt, stage max eigenvalues)."""

@@ -1,0 +1,214 @@
+"""Learned face reconstruction on the GPU against the oracle
+(oracle/learn_oracle.py, itself pinned by tests/test_learn_oracle.py).
+
+Bars: training gradients (fp32 kernels) within 1e-4 normwise of the float64
+oracle, loss within 1e-5; Adam within 1e-6 absolute; fp32 forward within
+1e-5; bf16 matrix-core forward within 5e-3 of max|y| of the oracle's bf16
+model (the device rounds fp32 sums, the oracle float64 ones, so a hidden
+activation can round to the neighbouring bf16 value); training rows built on
+the device bit-identical to the oracle's; face-state kernels with every cell on
+the plain fallback bit-identical to the order-2 gather kernel.
+"""
+import numpy as np
+import pytest
+
+from oracle import learn_oracle as LO
+
+pytestmark = pytest.mark.gpu
+
+
+def rows(seed, n):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1, 1, (n, LO.NIN)).astype(np.float32)
+    T = rng.uniform(-1, 1, (n, LO.NOUT)).astype(np.float32)
+    w = (rng.uniform(0, 1, n) > 0.2).astype(np.float32)
+    return X, T, w
+
+
+def test_loss_grad_matches_oracle(mmf):
+    net = mmf.Learner(seed=7)
+    p = net.params().astype(np.float64)
+    for n in (100, 32, 1):
+        X, T, w = rows(n, n)
+        w[0] = 1.0
+        loss, g = net.loss_grad(X, T, w)
+        lo, go = LO.loss_grad(p, X, T, w)
+        print("rows %d: loss %.8e vs %.8e, grad normwise %.3e" % (n, loss, lo, np.max(np.abs(g - go)) / np.max(np.abs(go))),
+              flush=True)
+        assert abs(loss - lo) <= 1e-5 * lo
+        assert np.max(np.abs(g - go)) <= 1e-4 * np.max(np.abs(go))
+
+
+def test_adam_matches_oracle(mmf):
+    net = mmf.Learner(seed=9)
+    p = net.params().astype(np.float64)
+    rng = np.random.default_rng(2)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in (1, 2, 3):
+        g = rng.normal(size=LO.NPARAMS).astype(np.float32)
+        net.adam_step(g, 1e-3)
+        p, m, v = LO.adam_step(p, g.astype(np.float64), m, v, step, 1e-3)
+    assert np.max(np.abs(net.params() - p)) <= 1e-6
+
+
+def test_forward_fp32_and_bf16_match_oracle(mmf):
+    net = mmf.Learner(seed=11)
+    p = net.params().astype(np.float64)
+    X, _, _ = rows(5, 1000)
+    y32 = net.forward(X, mfma=False)
+    yo = LO.forward(p, X)[0]
+    assert np.max(np.abs(y32 - yo)) <= 1e-5 * np.max(np.abs(yo))
+    yb = net.forward(X, mfma=True)
+    yob = LO.forward_bf16(p, X)
+    err = np.abs(yb - yob)
+    print("bf16: max %.3e, median %.3e (max|y| %.3f)" % (err.max(), np.median(err), np.abs(yob).max()), flush=True)
+    assert err.max() <= 5e-3 * np.abs(yob).max()
+    assert np.median(err) <= 1e-5 * np.abs(yob).max()
+    # the bf16 model approximates the fp32 one
+    assert np.max(np.abs(yb - yo)) <= 5e-2 * np.abs(yo).max()
+
+
+def test_forward_bf16_lane_layout(mmf):
+    """One-hot inputs through a network whose first layer is a permutation
+    pin which input reaches which hidden unit and which output each lane
+    writes: any transposed or shifted fragment gives a different row."""
+    net = mmf.Learner(seed=1)
+    P = LO.unpack(np.zeros(LO.NPARAMS))
+    perm = np.random.default_rng(4).permutation(LO.NIN)[:LO.NH]
+    for j in range(LO.NH):
+        P["W1"][j, perm[j]] = 1.0
+    P["W2"][:] = np.eye(LO.NH)
+    for o in range(LO.NOUT):
+        P["W3"][o, (5 * o + 3) % LO.NH] = 1.0
+    P["b3"][:] = np.arange(LO.NOUT) * 0.125
+    p = LO.pack(P).astype(np.float32)
+    net.set_params(p)
+    X = np.zeros((LO.NIN, LO.NIN), dtype=np.float32)
+    np.fill_diagonal(X, 2.0)
+    y = net.forward(X, mfma=True)
+    yo = LO.forward_bf16(p.astype(np.float64), X)
+    assert np.max(np.abs(y - yo)) <= 1e-6, np.argwhere(np.abs(y - yo) > 1e-6)[:5]
+
+
+def fine_state(mmf, key, n, steps, bc):
+    pb = mmf.problem(key, 2, vortex_periodic=1) if key == "vortex_xy" else \
+        mmf.problem(key, 2, domain_origin=[-1.0, -1.0, 0.0], domain_length=2.0, sod_r0=0.4)
+    s = mmf.Solver.for_problem(pb, n, order=2, side_bc=bc)
+    s.set_initial_state(pb)
+    s.set_time(0.0, 10.0, 0.5)
+    s.advance(steps)
+    return s
+
+
+@pytest.mark.parametrize("key,bc,periodic", [("vortex_xy", [4] * 4, (True, True)),
+                                             ("radsod", [0] * 4, (False, False))])
+def test_collect_and_restrict_match_oracle(mmf, key, bc, periodic):
+    n, r = 64, 4
+    s = fine_state(mmf, key, n, 25, bc)
+    U = s.get_state().reshape(n, n, 5)
+    assert np.array_equal(s.restrict_state(r), LO.restrict(U, r).reshape(-1, 5))
+    cells = [0, 5, 17, 63, 40, 136, 255, 119, 120]
+    net = mmf.Learner(capacity=1000)
+    net.collect(s, r, cells, 0.05)
+    X, T, w = net.samples()
+    Xo, To, wo = LO.collect(U, r, periodic, cells, 0.05)
+    print("%s: weights %s" % (key, w[::8].tolist()), flush=True)
+    assert np.array_equal(X, Xo)
+    assert np.array_equal(T, To)
+    assert np.array_equal(w, wo)
+
+
+def random_state(mmf, n, seed):
+    rng = np.random.default_rng(seed)
+    N = n[0] * n[1]
+    prim = np.empty((N, 5))
+    prim[:, 0] = 1.0 + 0.3 * rng.uniform(-1, 1, N)
+    prim[:, 1:3] = 0.3 * rng.uniform(-1, 1, (N, 2))
+    prim[:, 3] = 0.0
+    prim[:, 4] = 1.0 + 0.3 * rng.uniform(-1, 1, N)
+    return mmf.batch_prim2cons(prim)
+
+
+def run(s, U, steps=4):
+    s.reset_positivity_counters()
+    s.set_state(U)
+    R, me = s.compute_rhs()
+    s.set_time(0.0, 10.0, 0.5)
+    s.advance(steps)
+    return R, me, s.get_state(), s.time()[:3], s.positivity_counters()
+
+
+@pytest.mark.parametrize("bc", [[4, 4, 4, 4], [0, 1, 3, 2], [4, 4, 1, 1]])
+def test_face_state_kernels_equal_gather_order2(mmf, bc):
+    n = [50, 37]
+    info = [[1.0, 0.2, -0.1, 0.0, 1.0]] * 6
+    s = mmf.Solver(2, n, [-1.0, -1.0, 0.0], 0.04, bc + [0, 0], info, 2)
+    U = random_state(mmf, n, 3)
+    s.set_kernel(mmf.KERNEL_GATHER)
+    ref = run(s, U)
+    s.set_learned(None, enable=True)
+    got = run(s, U)
+    for a, b in zip(got, ref):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+    net = mmf.Learner(seed=1)
+    s.set_learned(net, sensor_threshold=-1.0)  # every cell's sensor fires: all plain
+    s.reset_learned_counts()
+    got = run(s, U)
+    for a, b in zip(got, ref):
+        if isinstance(a, np.ndarray):
+            assert np.array_equal(a, b)
+        else:
+            assert a == b
+    learned, fallback = s.learned_counts()
+    assert learned == 0 and fallback == n[0] * n[1] * (1 + 4)  # compute_rhs + 4 steps
+    s.set_learned(None, enable=False)
+    assert np.array_equal(run(s, U)[2], ref[2])
+
+
+def scaled_net(mmf, seed, scale):
+    net = mmf.Learner(seed=seed)
+    P = LO.unpack(net.params().astype(np.float64))
+    P["W3"] *= scale
+    p = LO.pack(P).astype(np.float32)
+    net.set_params(p)
+    return net, p.astype(np.float64)
+
+
+def test_learned_face_states_match_oracle(mmf):
+    n = 48
+    s = fine_state(mmf, "vortex_xy", n, 0, [4] * 4)
+    net, p = scaled_net(mmf, 3, 0.05)
+    s.set_learned(net, sensor_threshold=0.05)
+    F = s.face_states()
+    U = s.get_state().reshape(n, n, 5)
+    Fo, fbo = LO.learned_face_states(U, p, (True, True), 0.05)
+    Fo = Fo.reshape(-1, 4, 5)
+    fb = fbo.ravel()
+    _, c, sc, _ = LO.features(U, (True, True))
+    print("fallback cells (oracle): %d of %d" % (fb.sum(), fb.size), flush=True)
+    assert np.array_equal(F[fb], Fo[fb])
+    for v, k in enumerate(LO.VARS):
+        d = np.abs(F[~fb][:, :, k] - Fo[~fb][:, :, k])
+        assert np.all(d <= 1e-2 * sc[~fb][:, v:v + 1]), (v, d.max())
+    assert np.array_equal(F[:, :, 3], Fo[:, :, 3])
+
+
+def test_learned_run_conserves_mass(mmf):
+    n = 64
+    s = fine_state(mmf, "vortex_xy", n, 0, [4] * 4)
+    net, _ = scaled_net(mmf, 5, 0.05)
+    s.set_learned(net, sensor_threshold=0.05)
+    U0 = s.get_state()
+    s.set_time(0.0, 10.0, 0.5)
+    s.reset_learned_counts()
+    s.advance(20)
+    U1 = s.get_state()
+    learned, fallback = s.learned_counts()
+    assert learned > 0 and learned + fallback == 20 * n * n
+    assert np.all(np.isfinite(U1))
+    for k in (0, 1, 2, 4):
+        assert abs(U1[:, k].sum() - U0[:, k].sum()) <= 1e-12 * np.abs(U0[:, k]).sum(), k
