@@ -7,7 +7,9 @@ inside r<0.4 rho=1 p=1, outside rho=0.125 p=0.1, at rest, transmissive
 FP64.  --workload sphere192 runs the immersed-sphere case instead (192^3 on
 [-2,2]^3, sphere r=0.25, Mach 0.3 free stream, Dirichlet inflow at x-,
 free-flow elsewhere); --workload batch512 runs 512 independent seeded 2D
-cases together (persistent stage kernel, graph-replayed step).
+cases together (persistent stage kernel, graph-replayed step); --workload
+learned128 trains the learned face reconstruction on the device and compares
+its L1 density error and wall time with the plain limiter.
 
 A step is one SSP-RK3 time step (three fused flux+update stage kernels and
 the device-side dt/t update) over the whole grid, with the state resident
@@ -77,7 +79,8 @@ def sphere_setup(mmf, n, order):
 
 
 WORKLOADS = {"radsod256": (radsod_setup, 256, "radial_sod_3d"), "sphere192": (sphere_setup, 192, "sphere_3d"),
-             "batch512": (None, 512, "batch_2d_mixed")}
+             "batch512": (None, 512, "batch_2d_mixed"), "learned128": (None, 128, "learned_2d")}
+MLP_FLOPS_PER_CELL = 2 * (100 * 64 + 64 * 64 + 64 * 16)
 
 
 def cpu_baseline(n, steps):
@@ -215,6 +218,109 @@ def run_batch(args, mmf):
     print(json.dumps(out), flush=True)
 
 
+def cpu_baseline_order2(n, steps):
+    """The oracle (plain C, one thread): order-2 periodic vortex at n^2."""
+    from oracle import oracle as O
+    pb = O.problem(O.VORTEX_XY, 2, vortex_periodic=1)
+    s = O.Solver(pb, 2, [n, n], list(pb.domain_origin), pb.domain_length / n, side_bc=[O.BC_PERIODIC] * 4 + [0, 0])
+    U = s.initial_state()
+    W = U.copy()
+    t = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt, _ = s.rk3_step(U, W, 0.5, t, 100.0)
+        t += dt
+    el = time.perf_counter() - t0
+    return {"value": n * n * 3 * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d SSP-RK3 steps of the plain order-2 periodic vortex at %d^2 (limited linear, Rusanov), "
+                      "oracle/mmf_oracle.c -O2, single thread on %s; %.1f s" % (steps, n, cpu_model(), el)}
+
+
+def run_learned(args, mmf):
+    """Learned face reconstruction against the plain limiter: train on the
+    device from fine plain runs, then run held-out cases at n^2 (learned and
+    plain) and at 2n^2, 4n^2 (plain) to the same end time."""
+    import numpy as np
+    from minimmerflow_amd import learned as LR
+    n = args.n or 128
+    t0 = time.perf_counter()
+    train_cases = LR.case_params(args.train_cases, seed=11)
+    per_case = 40 * 64 * 8
+    net = mmf.Learner(seed=5, capacity=len(train_cases) * per_case)
+    LR.build_training_set(net, train_cases, 4 * n, 4, 40, 64, seed=1)
+    t_data = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    loss = None
+    for r in range(3):
+        loss = net.train(3000, batch=8192, lr0=2e-3 if r == 0 else 5e-4, lr1=2e-5, seed=r + 1)
+    t_train = time.perf_counter() - t1
+    log("training set %d rows from %d fine %d^2 runs (%.1fs); 9000 Adam steps, batch 8192 (%.1fs), loss %.3e"
+        % (net.num_samples(), len(train_cases), 4 * n, t_data, t_train, loss))
+    cases = []
+    ratios, speed256, speed512 = [], [], []
+    for c in LR.case_params(args.eval_cases, seed=99):
+        runs = {}
+        for m in (n, 2 * n, 4 * n):
+            runs["plain%d" % m] = LR.run_case(c, m)
+        runs["learned%d" % n] = LR.run_case(c, n, net=net)
+        ref = runs["plain%d" % (4 * n)][0]
+        out = {"case": c}
+        for key, (sv, pb, steps, wall) in runs.items():
+            m = sv.n[0]
+            out[key] = {"l1": LR.l1_error(c, sv, pb, ref, 4 * n // m), "steps": steps, "wall_s": wall}
+        lc, fc = runs["learned%d" % n][0].learned_counts()
+        out["network_fraction"] = lc / max(1, lc + fc)
+        L, P1 = out["learned%d" % n], out["plain%d" % n]
+        ratios.append(P1["l1"] / L["l1"])
+        speed256.append(out["plain%d" % (2 * n)]["wall_s"] / L["wall_s"])
+        speed512.append(out["plain%d" % (4 * n)]["wall_s"] / L["wall_s"])
+        log("%s: L1 plain %d^2 %.5f, %d^2 %.5f, %d^2 %.5f | learned %d^2 %.5f; wall learned %.4fs vs plain %d^2 %.4fs"
+            % (c["kind"], n, P1["l1"], 2 * n, out["plain%d" % (2 * n)]["l1"], 4 * n, out["plain%d" % (4 * n)]["l1"],
+               n, L["l1"], L["wall_s"], 2 * n, out["plain%d" % (2 * n)]["wall_s"]))
+        cases.append(out)
+    # stage timing of the learned reconstruction + flux kernels (HIP events)
+    c = LR.case_params(1, seed=99)[0]
+    sv, _ = LR.solver(c, n)
+    sv.set_learned(net, 0.05)
+    sv.set_time(0.0, 1e30, LR.CFL)
+    sv.advance(3)
+    sv.enable_timing(True)
+    sv.advance(20)
+    stage_ms, launches = sv.read_timing()
+    sv.enable_timing(False)
+    avg = stage_ms / max(1, launches)
+    flops = MLP_FLOPS_PER_CELL * n * n
+    achieved = flops / (avg * 1e-3) / 1e12
+    cpu = None if args.no_cpu_baseline else cpu_baseline_order2(n, 40)
+    gm = float(np.exp(np.mean(np.log(ratios))))
+    out = {
+        "metric": "L1 rho error at equal time, learned vs plain (plain / learned at %d^2, geometric mean)" % n,
+        "value": gm,
+        "unit": "x (error reduction)",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": None,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (network bf16 in, fp32 accumulate)",
+        "data": "synthetic (seeded initial conditions); training rows built on the device from fine plain runs",
+        "config": {"workload": "learned_2d_%d" % n, "train_cases": len(train_cases), "fine": 4 * n,
+                   "training_rows": net.num_samples(), "training_s": t_train, "data_s": t_data, "final_loss": loss,
+                   "cases": cases,
+                   "wall_speedup_vs_plain_2n": float(np.exp(np.mean(np.log(speed256)))),
+                   "wall_speedup_vs_plain_4n": float(np.exp(np.mean(np.log(speed512)))),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": achieved / 2500.0, "traffic": None,
+                     "kernel": "k_recon_learned + k_flux_fs (one RK stage), network FLOPs only",
+                     "algorithmic_flops_per_launch": flops, "avg_launch_ms": avg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if os.path.exists(p):
@@ -238,6 +344,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
     ap.add_argument("--ncases", type=int, default=512, help="batch: number of cases")
+    ap.add_argument("--train-cases", type=int, default=16, help="learned: fine training runs")
+    ap.add_argument("--eval-cases", type=int, default=4, help="learned: held-out cases")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -259,6 +367,9 @@ def main():
 
     if args.workload == "batch512":
         run_batch(args, mmf)
+        return
+    if args.workload == "learned128":
+        run_learned(args, mmf)
         return
 
     setup, n_default, wname = WORKLOADS[args.workload]

@@ -123,6 +123,10 @@ int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl);
 /* Enqueue nsteps SSP-RK3 steps; dt and t stay on the device and a step
  * whose start time is >= tmax does nothing.  Does not synchronise. */
 int mmf_advance(mmf_ctx *ctx, int64_t nsteps);
+/* Steps are replayed from HIP graphs captured on first use (re-captured
+ * when the kernel choice, bodies or reconstruction change); on 0 they are
+ * launched one by one.  Results are identical either way. */
+int mmf_set_step_graphs(mmf_ctx *ctx, int32_t on);
 /* Step until t >= tmax (or max_steps when >= 0); synchronises. */
 int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_final);
 /* Synchronises and reads the time control block.  stage_eig: the three

@@ -98,6 +98,7 @@ def lib():
         "mmf_compute_rhs": ([vp, dp, dp], C.c_int),
         "mmf_set_time": ([vp, C.c_double, C.c_double, C.c_double], C.c_int),
         "mmf_advance": ([vp, C.c_int64], C.c_int),
+        "mmf_set_step_graphs": ([vp, C.c_int32], C.c_int),
         "mmf_run": ([vp, C.c_int64, i64p, dp], C.c_int),
         "mmf_get_time": ([vp, dp, dp, i64p, dp], C.c_int),
         "mmf_synchronize": ([vp], C.c_int),
@@ -405,6 +406,9 @@ class Solver:
 
     def advance(self, nsteps):
         self._c(lib().mmf_advance(self.ctx, int(nsteps)))
+
+    def set_step_graphs(self, on=True):
+        self._c(lib().mmf_set_step_graphs(self.ctx, 1 if on else 0))
 
     def run(self, max_steps=-1):
         steps, t = C.c_int64(), C.c_double()

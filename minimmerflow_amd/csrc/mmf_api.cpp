@@ -224,6 +224,7 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->hctrl) (void)hipHostFree(ctx->hctrl);
   if (ctx->dpartials) (void)hipFree(ctx->dpartials);
   if (ctx->derr) (void)hipFree(ctx->derr);
+  mmf_drop_step_graphs(ctx);
   if (ctx->dFS) (void)hipFree(ctx->dFS);
   if (ctx->dlcount) (void)hipFree(ctx->dlcount);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -434,8 +435,11 @@ int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl) {
   return MMF_OK;
 }
 
-int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
-  if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
+namespace {
+
+constexpr int GRAPH_STEPS = 16;
+
+int enqueue_steps(mmf_ctx *ctx, int64_t nsteps) {
   for (int64_t s = 0; s < nsteps; ++s) {
     for (int stage = 1; stage <= 3; ++stage) {
       int rc = launch_stage_timed(ctx, stage, mmf::MODE_STEP);
@@ -444,6 +448,83 @@ int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
     hipError_t e = mmf::launch_advance(ctx->dctrl, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "advance kernel");
   }
+  return MMF_OK;
+}
+
+// everything the captured launches depend on (kernel arguments by value)
+std::vector<char> launch_key(const mmf_ctx *ctx) {
+  std::vector<char> k(sizeof(GridDev) + sizeof(int) + sizeof(mmf::MarchShape) + sizeof(bool) + sizeof(mmf::LearnRun));
+  char *p = k.data();
+  std::memcpy(p, &ctx->g, sizeof(GridDev));
+  p += sizeof(GridDev);
+  std::memcpy(p, &ctx->kernel, sizeof(int));
+  p += sizeof(int);
+  std::memcpy(p, &ctx->shape, sizeof(mmf::MarchShape));
+  p += sizeof(mmf::MarchShape);
+  std::memcpy(p, &ctx->learn_on, sizeof(bool));
+  p += sizeof(bool);
+  std::memcpy(p, &ctx->learn, sizeof(mmf::LearnRun));
+  return k;
+}
+
+int capture_steps(mmf_ctx *ctx, int slot, int64_t nsteps) {
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_steps(ctx, nsteps);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != MMF_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "end of step capture");
+  ctx->step_graph[slot] = g;
+  HIPCHK(ctx, hipGraphInstantiate(&ctx->step_exec[slot], g, nullptr, nullptr, 0));
+  return MMF_OK;
+}
+
+}  // namespace
+
+void mmf_drop_step_graphs(mmf_ctx *ctx) {
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->step_exec[i]) (void)hipGraphExecDestroy(ctx->step_exec[i]);
+    if (ctx->step_graph[i]) (void)hipGraphDestroy(ctx->step_graph[i]);
+    ctx->step_exec[i] = nullptr;
+    ctx->step_graph[i] = nullptr;
+  }
+  ctx->graph_key.clear();
+}
+
+// Steps are replayed from captured HIP graphs (GRAPH_STEPS steps per replay,
+// then single steps), re-captured whenever the launch configuration changed;
+// with kernel timing on they are launched one by one.
+int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
+  if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
+  if (ctx->timing || !ctx->use_graphs || nsteps == 0) return enqueue_steps(ctx, nsteps);
+  std::vector<char> key = launch_key(ctx);
+  if (key != ctx->graph_key) {
+    mmf_drop_step_graphs(ctx);
+    ctx->graph_key = key;
+  }
+  while (nsteps > 0) {
+    const int slot = (nsteps >= GRAPH_STEPS) ? 1 : 0;
+    if (!ctx->step_exec[slot]) {
+      int rc = capture_steps(ctx, slot, slot ? GRAPH_STEPS : 1);
+      if (rc != MMF_OK) {
+        mmf_drop_step_graphs(ctx);
+        return rc;
+      }
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->step_exec[slot], ctx->stream));
+    nsteps -= slot ? GRAPH_STEPS : 1;
+  }
+  return MMF_OK;
+}
+
+int mmf_set_step_graphs(mmf_ctx *ctx, int32_t on) {
+  if (!ctx) return MMF_ERR_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->use_graphs = on != 0;
+  if (!ctx->use_graphs) mmf_drop_step_graphs(ctx);
   return MMF_OK;
 }
 

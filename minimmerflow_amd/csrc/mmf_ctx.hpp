@@ -39,6 +39,15 @@ struct mmf_ctx {
   bool learn_on = false;
   mmf::LearnRun learn{};
   bool learn_opts_set = false;
+  // captured step graphs (1 step, GRAPH_STEPS steps) and the launch
+  // configuration they were captured with
+  hipGraph_t step_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+  std::vector<char> graph_key;
+  bool use_graphs = true;
   double *dFS = nullptr;
   unsigned long long *dlcount = nullptr;
 };
+
+// releases the captured step graphs of a context (defined with the C-ABI)
+extern "C" void mmf_drop_step_graphs(mmf_ctx *ctx);

@@ -361,15 +361,27 @@ hipError_t launch_learn_train(const float *params, const float *X, const float *
   return hipGetLastError();
 }
 
-// gradient = sum of the block partials in block order; loss = sum of the
-// block loss sums / (16 rows), in block order, in fp64
+// gradient = sum of the block partials: each of RSPLIT groups of blocks is
+// summed in block order by one thread, the group sums in group order by a
+// second; loss = block loss sums in block order, in fp64, / (16 rows)
+constexpr int RSPLIT = 16;
+
 __global__ __launch_bounds__(256) void k_learn_reduce(const float *partial, const float *ploss, int64_t nblocks,
                                                       int64_t rows, float *grad, double *loss) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p < NPARAMS) {
-    float a = 0.f;
-    for (int64_t b = 0; b < nblocks; ++b) a += partial[b * NPARAMS + p];
-    grad[p] = a;
+  __shared__ float part[RSPLIT][16];
+  const int pl = threadIdx.x & 15, grp = threadIdx.x >> 4;  // 16 parameters x 16 groups per block
+  const int p = blockIdx.x * 16 + pl;
+  const int64_t per = (nblocks + RSPLIT - 1) / RSPLIT;
+  const int64_t b0 = grp * per, b1 = (b0 + per < nblocks) ? b0 + per : nblocks;
+  float a = 0.f;
+  if (p < NPARAMS)
+    for (int64_t b = b0; b < b1; ++b) a += partial[b * NPARAMS + p];
+  part[grp][pl] = a;
+  __syncthreads();
+  if (grp == 0 && p < NPARAMS) {
+    float t = 0.f;
+    for (int g = 0; g < RSPLIT; ++g) t += part[g][pl];
+    grad[p] = t;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0 && loss) {
     double l = 0.0;
@@ -380,7 +392,7 @@ __global__ __launch_bounds__(256) void k_learn_reduce(const float *partial, cons
 
 hipError_t launch_learn_reduce(const float *partial, const float *ploss, int64_t nblocks, int64_t rows, float *grad,
                                double *loss, hipStream_t s) {
-  hipLaunchKernelGGL(k_learn_reduce, dim3((NPARAMS + 255) / 256), dim3(256), 0, s, partial, ploss, nblocks, rows, grad,
+  hipLaunchKernelGGL(k_learn_reduce, dim3((NPARAMS + 15) / 16), dim3(256), 0, s, partial, ploss, nblocks, rows, grad,
                      loss);
   return hipGetLastError();
 }
@@ -576,7 +588,8 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   __shared__ uint8_t rex[RGY][RGX];
   __shared__ s4 sF[NFRAG * 64];
   __shared__ float sB[2 * NH + NOUT];
-  __shared__ double scs[RTY][16][4][2];
+  __shared__ double sc[RTY][16][4];  // centre value per cell and stencil variable
+  __shared__ float sis[RTY][16][4];   // 1 / scale
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
   const double dt = (STAGE == 2) ? step_dt(ctrl) : 0.0;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
@@ -624,11 +637,11 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
         dev = (d > dev) ? d : dev;
       }
     const double tau = TAU_REL * (fabs(reg[FID_RHO][ry][rx]) + fabs(reg[FID_RHO_E][ry][rx]));
-    scs[w][c][grp][0] = cv;
-    scs[w][c][grp][1] = (dev >= tau) ? dev : tau;
-    // shock sensor: lane group 0 on rho, group 1 on p
+    sc[w][c][grp] = cv;
+    sis[w][c][grp] = (float)(1.0 / ((dev >= tau) ? dev : tau));
+    // shock sensor: lane group 0 on rho, group 1 on p; fires where
+    // |q+ - 2 q0 + q-| > theta (q+ + 2 q0 + q-) along either axis
     if (grp < 2) {
-      double phi = 0.0;
       for (int a = 0; a < 2; ++a) {
         double q[3];
         for (int t = 0; t < 3; ++t) {
@@ -636,24 +649,22 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
           if (grp == 0) {
             q[t] = reg[FID_RHO][yy][xx];
           } else {
-            double c5[NF], p5[NF];
-#pragma unroll
-            for (int k = 0; k < NF; ++k) c5[k] = reg[k][yy][xx];
-            cons2prim(c5, p5);
-            q[t] = p5[FID_P];
+            // p = (gamma - 1) (rho E - |rho u|^2 / (2 rho))
+            const double r0 = reg[FID_RHO][yy][xx], mx = reg[FID_RHO_U][yy][xx], my = reg[FID_RHO_V][yy][xx],
+                         mz = reg[FID_RHO_W][yy][xx];
+            q[t] = (GAMMA - 1.0) * (reg[FID_RHO_E][yy][xx] - 0.5 * ((mx * mx + my * my) + mz * mz) / r0);
           }
         }
-        const double val = fabs((q[2] - 2.0 * q[1]) + q[0]) / ((q[2] + 2.0 * q[1]) + q[0]);
-        phi = (val > phi) ? val : phi;
+        const double num = fabs((q[2] - 2.0 * q[1]) + q[0]);
+        const double den = (q[2] + 2.0 * q[1]) + q[0];
+        fires = fires || !(den > 0.0) || !(num <= run.theta * den);
       }
-      fires = !(phi <= run.theta);
     }
   }
   __syncthreads();
 
   // plain limited linear face value of face grp (reconstruction.cpp:67-97)
-  double P[NF];
-  {
+  auto plain_face = [&](double *P) {
     const int a = grp >> 1, side = grp & 1;
     const int my = ry - a, mx = rx - (1 - a), py = ry + a, px = rx + (1 - a);
     const bool em = rex[my][mx], ep = rex[py][px];
@@ -671,7 +682,10 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
 #pragma unroll
       for (int k = 0; k < NF; ++k) P[k] = X0[k];
     }
-  }
+  };
+  double P[NF];
+  const bool need_plain_all = run.blend != 1.0;
+  if (need_plain_all) plain_face(P);
 
   double Fv[NF];
   bool pos = false;
@@ -686,7 +700,7 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
         if (live && k < NIN) {
           const int v = k / 25, rr = k % 25;
           const double S = reg[var_of(v)][ry - 2 + rr / 5][rx - 2 + rr % 5];
-          xv = (float)((S - scs[w][c][v][0]) / scs[w][c][v][1]);
+          xv = (float)(S - sc[w][c][v]) * sis[w][c][v];
         }
         xb[kb][i] = f2bf(xv);
       }
@@ -706,7 +720,7 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
         u[d + 2] = a ? reg[k][ry + dd][rx] : reg[k][ry][rx + dd];
       }
       const double Pm = u[2] + 0.5 * minmod(u[3] - u[2], u[2] - u[1]);
-      const double Q5 = ((((2.0 * u[0] - 13.0 * u[1]) + 47.0 * u[2]) + 27.0 * u[3]) - 3.0 * u[4]) / 60.0;
+      const double Q5 = ((((2.0 * u[0] - 13.0 * u[1]) + 47.0 * u[2]) + 27.0 * u[3]) - 3.0 * u[4]) * (1.0 / 60.0);
       const double sg = (double)(1.0f / (1.0f + expf(-y[v])));
       double f = Pm + sg * (Q5 - Pm);
       if (run.clip) {
@@ -719,9 +733,9 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
       Fv[k] = f;
     }
     Fv[FID_RHO_W] = reg[FID_RHO_W][ry][rx];
-    double pr[NF];
-    cons2prim(Fv, pr);
-    pos = (Fv[FID_RHO] > 0.0) && (pr[FID_P] > 0.0);
+    // rho > 0 and p > 0, with p > 0 <=> 2 rho (rho E) > |rho u|^2 for rho > 0
+    const double m2 = (Fv[FID_RHO_U] * Fv[FID_RHO_U] + Fv[FID_RHO_V] * Fv[FID_RHO_V]) + Fv[FID_RHO_W] * Fv[FID_RHO_W];
+    pos = (Fv[FID_RHO] > 0.0) && (2.0 * Fv[FID_RHO] * Fv[FID_RHO_E] > m2);
   }
   const unsigned long long fire_m = __ballot(fires);
   const unsigned long long bad_m = __ballot(!pos);
@@ -730,6 +744,7 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   const bool learned = use_net && learnable && !cell_fire && !cell_bad;
 
   if (live && !learned) {
+    if (!need_plain_all) plain_face(P);
 #pragma unroll
     for (int k = 0; k < NF; ++k) Fv[k] = P[k];
   }

@@ -212,3 +212,26 @@ def test_learned_run_conserves_mass(mmf):
     assert np.all(np.isfinite(U1))
     for k in (0, 1, 2, 4):
         assert abs(U1[:, k].sum() - U0[:, k].sum()) <= 1e-12 * np.abs(U0[:, k]).sum(), k
+
+
+def test_trained_network_gains_over_plain(mmf):
+    """Train on the device (fine 256^2 plain runs restricted 4x, seeded vortex
+    and radial-Sod cases) and run held-out cases at 64^2: the learned face
+    values must cut the L1 density error of the plain limiter (vortex against
+    the exact solution, Sod against the fine run restricted)."""
+    from minimmerflow_amd import learned as LR
+    cases = LR.case_params(8, seed=11)
+    net = mmf.Learner(seed=5, capacity=8 * 40 * 64 * 8)
+    LR.build_training_set(net, cases, 256, 4, 40, 64, seed=1)
+    loss = net.train(1500, batch=8192, lr0=2e-3, lr1=2e-5, seed=1)
+    print("training rows %d, final loss %.3e" % (net.num_samples(), loss), flush=True)
+    for c in LR.case_params(2, seed=99):
+        ref, _, _, _ = LR.run_case(c, 256)
+        plain, pb, _, _ = LR.run_case(c, 64)
+        learned, pbl, _, _ = LR.run_case(c, 64, net=net)
+        ep = LR.l1_error(c, plain, pb, ref, 4)
+        el = LR.l1_error(c, learned, pbl, ref, 4)
+        lc, fc = learned.learned_counts()
+        print("%s: plain %.5f learned %.5f (network on %.1f%% of cell-stages)"
+              % (c["kind"], ep, el, 100.0 * lc / max(1, lc + fc)), flush=True)
+        assert el < (0.5 if c["kind"] == "vortex" else 0.9) * ep

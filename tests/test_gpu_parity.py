@@ -295,3 +295,30 @@ def test_sphere_freestream_steps(mmf, orc):
     print("sphere 8 steps: normwise %.3e bit-identical %.6f" % (err, float(np.mean(Ug == U))), flush=True)
     assert tg == t and stepg == 8
     assert err <= TOL
+
+
+@pytest.mark.parametrize("key,dim,nc,order", [("vortex_xy", 2, 40, 2), ("radsod", 3, 24, 1), ("radsod", 2, 64, 1)])
+def test_graph_replayed_steps_equal_launched_steps(mmf, key, dim, nc, order):
+    """Steps replayed from captured graphs (16-step and 1-step graphs, with
+    host reads, a state upload and a kernel change between replays) equal
+    steps launched one by one."""
+    pm = mmf.problem(key, dim)
+    runs = []
+    for graphs in (False, True):
+        s = mmf.Solver.for_problem(pm, nc, order=order)
+        s.set_step_graphs(graphs)
+        s.set_initial_state(pm)
+        s.set_time(0.0, 100.0, 0.45)
+        states = []
+        for n in (17, 1, 3, 16):
+            s.advance(n)
+            states.append(s.get_state())
+            s.set_state(states[-1])
+            t, dt, step, _ = s.time()
+            states.append(np.array([t, dt, step]))
+        s.set_kernel(mmf.KERNEL_GATHER)
+        s.advance(5)
+        states.append(s.get_state())
+        runs.append(states)
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
