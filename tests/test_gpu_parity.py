@@ -322,3 +322,29 @@ def test_graph_replayed_steps_equal_launched_steps(mmf, key, dim, nc, order):
         runs.append(states)
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+def test_vortex_64_periodic_order2_to_t10(mmf, orc):
+    """The reference CPU configuration end to end: 2D isentropic vortex, 64^2 on
+    [-5,5]^2, periodic, beta 5, limited linear + Rusanov, SSP-RK3 at CFL 0.5 to
+    t = 10 (one full period of the free stream).  GPU and oracle take the same
+    number of steps to the same time and end on the same state."""
+    pm = mmf.problem("vortex_xy", 2, vortex_periodic=1)
+    po = orc.problem(orc.VORTEX_XY, 2, vortex_periodic=1)
+    assert pm.vortex_beta == 5.0
+    s = mmf.Solver.for_problem(pm, 64, order=2, side_bc=[mmf.BC_PERIODIC] * 4)
+    s.set_initial_state(pm)
+    s.set_time(0.0, 10.0, 0.5)
+    steps, t = s.run()
+    o = orc.Solver(po, 2, [64, 64], list(po.domain_origin), po.domain_length / 64, side_bc=[4] * 4 + [0, 0])
+    U = o.initial_state()
+    so, to = o.run(U, 0.5, 0.0, 10.0)
+    G = s.get_state()
+    err = np.max(np.abs(G - U)) / np.max(np.abs(U))
+    l1g, _ = s.error_vs_exact(pm, 10.0)
+    l1o, _ = o.error(U, 10.0)
+    print("steps %d/%d t %.15f/%.15f normwise %.3e identical %.4f L1 %.12e / %.12e"
+          % (steps, so, t, to, err, np.mean(G == U), l1g, l1o), flush=True)
+    assert steps == so and t == to
+    assert err <= 1e-12
+    assert abs(l1g - l1o) <= 1e-12 * l1o
