@@ -440,16 +440,17 @@ hipError_t launch_batch_bc_values(int bc, const double *nrm, const double *info,
 }
 
 // plain copy kernel for HBM byte-counter calibration
-__global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ src, double4 *__restrict__ dst, int64_t n4) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+// one 16-byte element per thread: the most memory-level parallelism per byte
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const double4 *src, double4 *dst, int64_t n4, hipStream_t s) {
-  int64_t blocks = (n4 + 255) / 256;
-  if (blocks > 256 * 64) blocks = 256 * 64;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n4);
+  const int64_t n2 = 2 * n4;
+  const int64_t blocks = (n2 + 255) / 256;
+  if (blocks < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const double2 *)src, (double2 *)dst, n2);
   return hipGetLastError();
 }
 
