@@ -84,8 +84,8 @@ int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *ninterfaces);
 
 /* ---- stage-kernel selection ------------------------------------------------ */
 enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2 };
-/* AUTO: the marching LDS-tile kernel wherever it applies (order 1), else the
- * per-cell gather kernel.  tile_rows (3D: 4 or 8) and chunk (planes per
+/* AUTO: the marching LDS-tile kernels wherever they apply (order 1; order 2
+ * in 2D without bodies), else the per-cell gather kernel.  tile_rows (3D: 4 or 8) and chunk (planes per
  * block along the last axis) shape the marching kernel; <= 0 keeps the
  * default.  Every variant gives bit-identical results. */
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk);

@@ -69,6 +69,15 @@ void soa_to_aos(const mmf_ctx *ctx, const std::vector<double> &soa, double *aos)
   }
 }
 
+// Kernel actually used: MARCH where it applies; AUTO takes it as well except
+// for 2D order-2 grids below 2^20 cells, where the per-cell gather kernel is
+// faster (the strip kernel's row march is latency-bound at one wave per SIMD).
+bool use_march(const mmf_ctx *ctx) {
+  if (ctx->kernel == mmf::KERNEL_GATHER || !mmf::march_shape_supported(ctx->g, ctx->shape)) return false;
+  if (ctx->kernel == mmf::KERNEL_AUTO && ctx->g.order == 2 && ctx->g.N < ((int64_t)1 << 20)) return false;
+  return true;
+}
+
 int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->timing) {
@@ -84,7 +93,7 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
       e = mmf::launch_flux_fs(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
                               ctx->dctrl, ctx->stream);
   } else {
-    const bool march = ctx->kernel != mmf::KERNEL_GATHER && mmf::march_shape_supported(ctx->g, ctx->shape);
+    const bool march = use_march(ctx);
     e = march ? mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW,
                                         ctx->dU, ctx->dctrl, ctx->stream)
               : mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
@@ -239,7 +248,7 @@ int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk)
   if (tile_rows > 0) sh.ty = tile_rows;
   if (chunk > 0) sh.chunk = chunk;
   if (kind == mmf::KERNEL_MARCH && !mmf::march_shape_supported(ctx->g, sh))
-    return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1 only, fewer than 2^29 cells; tile rows 4 or 8 in 3D, 1 in 2D");
+    return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1, or order 2 in 2D without bodies; fewer than 2^29 cells; tile rows 4 or 8 in 3D, 1 in 2D");
   ctx->kernel = kind;
   ctx->shape = sh;
   return MMF_OK;
@@ -247,7 +256,7 @@ int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk)
 
 int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk) {
   if (!ctx) return MMF_ERR_INVALID;
-  const bool march = ctx->kernel != mmf::KERNEL_GATHER && mmf::march_shape_supported(ctx->g, ctx->shape);
+  const bool march = use_march(ctx);
   if (kind) *kind = march ? mmf::KERNEL_MARCH : mmf::KERNEL_GATHER;
   if (tile_rows) *tile_rows = ctx->shape.ty;
   if (chunk) *chunk = ctx->shape.chunk;

@@ -20,6 +20,8 @@ struct BatchCase {
 
 hipError_t launch_stage_march2d(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
                                 const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s);
+hipError_t launch_stage_march2d_o2(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
+                                   const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s);
 int march2d_tiles(int64_t nx);
 hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,
                               double *U, double *R, double *W, int nblocks, hipStream_t s);

@@ -534,7 +534,9 @@ static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int sta
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
-  if (g.order != 1 || sh.chunk < 1 || g.N >= MARCH_MAX_CELLS) return false;
+  if (sh.chunk < 1 || g.N >= MARCH_MAX_CELLS) return false;
+  // order 2: the 2D strip kernel, grids without bodies
+  if (g.order != 1 && !(g.order == 2 && g.dim == 2 && g.flags == nullptr)) return false;
   if (g.dim == 2) return sh.ty == 1;
   return sh.ty == 4 || sh.ty == 8;
 }
@@ -558,7 +560,10 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &sh, int stage,
                               const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                               hipStream_t s) {
   const bool mask = g.flags != nullptr;
-  if (g.dim == 2) return launch_stage_march2d(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  if (g.dim == 2) {
+    if (g.order == 2) return launch_stage_march2d_o2(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return launch_stage_march2d(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
   switch (sh.ty) {
     case 4:
       return mask ? launch_march_s<3, 4, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)

@@ -409,6 +409,306 @@ hipError_t launch_stage_march2d(const GridDev &g, int chunk, int stage, int mode
 
 int march2d_tiles(int64_t nx) { return (int)((nx + INNER - 1) / INNER); }
 
+// ---- order 2 (limited linear) --------------------------------------------------
+// One wave per item: a strip of 60 cells along x plus two halo lanes on each
+// side (lanes 0, 1 and w+2, w+3; x0 = 60 * tile), marching along y over a
+// chunk of rows with a window of three rows of the stage state in registers.
+// For each row c: the y slope of the column (rows c-1, c, c+1) gives the
+// cell's lower and upper y face values; the y face between rows c-1 and c is
+// evaluated once, from the saved upper value of row c-1 and the lower value
+// of row c; the row is staged in LDS for the x slopes, and each x face is
+// evaluated once, by the lane on its lower side, from its upper x value and
+// the next lane's lower x value (through LDS).  A row is completed one step
+// later, when its upper y face is known.  Slopes, face values with their
+// positivity fallback (reconstruction.cpp:67-97), border states and the
+// residual order are those of the gather kernel: results are bit-identical.
+constexpr int W2O = 64;
+constexpr int INNER2 = W2O - 4;
+
+namespace {
+
+__device__ __forceinline__ void recon_side(const double *mean, const double *sl, bool upper, double *v) {
+#pragma unroll
+  for (int k = 0; k < NF; ++k) v[k] = upper ? mean[k] + 0.5 * sl[k] : mean[k] - 0.5 * sl[k];
+  double pr[NF];
+  cons2prim(v, pr);
+  if (!(v[FID_RHO] > 0.0) || !(pr[FID_P] > 0.0)) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) v[k] = mean[k];
+  }
+}
+
+// Area-weighted flux of a border face from the fluid side's face value.
+__device__ __forceinline__ double border_face2(const GridDev &g, int side, int a, const Tup &Fl, bool fluid_lower,
+                                               double *AF, Ctrl *ctrl) {
+  double v[NF];
+  border_state(g, side, a, Fl.c, v);
+  Tup V;
+  make_tup(v, V);
+  double lam;
+  if (fluid_lower) {
+    lam = face_flux_area(Fl, V, a, g.pow2A, g.A, g.halfA, AF);
+    count_neg(Fl, V, ctrl);
+  } else {
+    lam = face_flux_area(V, Fl, a, g.pow2A, g.A, g.halfA, AF);
+    count_neg(V, Fl, ctrl);
+  }
+  return lam;
+}
+
+}  // namespace
+
+template <int STAGE, bool P2>
+__global__ __launch_bounds__(W2O) void k_march2d_o2(GridDev g, const double *X, const double *Uin, const double *R,
+                                                    double *out, Ctrl *ctrl, int mode, int chunk, int ntx) {
+  __shared__ double srow[NF][W2O];   // the row's states, for the x slopes
+  __shared__ double sfl[NTUP][W2O];  // each lane's lower x face value (tuple)
+  __shared__ double saf[NF][W2O];    // each lane's upper x face flux
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;
+  const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+  GridDev gg = g;
+  gg.pow2A = P2;
+  gg.pow2V = P2;
+  const double A = g.A, halfA = g.halfA;
+  const int lane = threadIdx.x;
+  const int tile = blockIdx.x % ntx, bz = blockIdx.x / ntx;
+  const int nx = (int)g.n[0], ny = (int)g.n[1];
+  const int m0 = bz * chunk;
+  const int m1 = (m0 + chunk < ny) ? m0 + chunk : ny;
+  const int x0 = tile * INNER2;
+  const int w = (nx - x0 < INNER2) ? nx - x0 : INNER2;
+  const int64_t fs = g.fs;
+  auto col_of = [&](int l, bool &e) -> int {
+    int c = x0 - 2 + l;
+    e = l >= 0 && l <= w + 3;
+    if (c < 0 || c >= nx) {
+      if (g.periodic[0]) c = ((c % nx) + nx) % nx;
+      else e = false;
+    }
+    return c;
+  };
+  bool ex, exl, exr;
+  const int cx = col_of(lane, ex);
+  (void)col_of(lane - 1, exl);
+  (void)col_of(lane + 1, exr);
+  const bool interior = lane >= 2 && lane <= w + 1;
+  const bool owner = lane >= 1 && lane <= w + 1;  // x face between this column and the next
+  const bool wrap_x = g.periodic[0] && (x0 - 2 + lane) == nx - 1;
+  const uint32_t col = (uint32_t)cx;
+  auto row_of = [&](int c, bool &e) -> int {
+    e = true;
+    if (c < 0 || c >= ny) {
+      if (g.periodic[1]) return ((c % ny) + ny) % ny;
+      e = false;
+      return 0;
+    }
+    return c;
+  };
+  In2<STAGE, P2> in{X, Uin, R, fs};
+  const int last = m1 + 1;  // last row loaded
+
+  // window: rows c-1 (a), c (b), c+1 (cc) of this column
+  double Xa[NF], Xb[NF], Xc[NF];
+  bool ea = false, eb = false, ec = false;
+  {
+    bool e;
+    int r = row_of(m0 - 2, e);
+    ea = e && ex;
+    if (ea) in.state(gg, dt, col + (uint32_t)r * (uint32_t)nx, Xa);
+    r = row_of(m0 - 1, e);
+    eb = e && ex;
+    if (eb) in.state(gg, dt, col + (uint32_t)r * (uint32_t)nx, Xb);
+    r = row_of(m0, e);
+    ec = e && ex;
+    if (ec) in.state(gg, dt, col + (uint32_t)r * (uint32_t)nx, Xc);
+  }
+  Pf2<STAGE> pf;
+  bool epf = false;
+  {
+    bool e;
+    const int r = row_of(m0 + 1, e);
+    epf = e && ex && (m0 + 1 <= last);
+    if (epf) pf.issue(in, col + (uint32_t)r * (uint32_t)nx);
+  }
+
+  double lam_max = 0.0;
+  Tup TyU;                // upper y face value of row c-1
+  double px[NF], pym[NF];  // pending row: x part of its residual, its lower y face
+#pragma unroll
+  for (int k = 0; k < NF; ++k) px[k] = pym[k] = 0.0;
+
+  for (int c = m0 - 1; c <= m1; ++c) {
+    // U of row c-1, completed in this step (stages 2, 3)
+    double Uc1[NF];
+    if (STAGE >= 2 && interior && c - 1 >= m0) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) Uc1[k] = ld(Uin, fs, k, col + (uint32_t)(c - 1) * (uint32_t)nx);
+    }
+    // y face values of row c
+    double FyL[NF], FyU[NF];
+    if (eb) {
+      double sy[NF];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        const double dm = ea ? Xb[k] - Xa[k] : 0.0;
+        const double dp = ec ? Xc[k] - Xb[k] : 0.0;
+        sy[k] = minmod(dp, dm);
+      }
+      recon_side(Xb, sy, false, FyL);
+      recon_side(Xb, sy, true, FyU);
+    }
+    // y face between rows c-1 and c, once per face
+    double AFy[NF];
+    if (interior && c >= m0) {
+      double l;
+      if (ea && eb) {
+        Tup TL;
+        make_tup(FyL, TL);
+        l = face_flux_area(TyU, TL, 1, P2, A, halfA, AFy);
+        if (tup_negative(TyU) | tup_negative(TL)) count_neg(TyU, TL, ctrl);
+      } else if (eb) {  // lower border of the y axis
+        Tup TL;
+        make_tup(FyL, TL);
+        l = border_face2(gg, 2, 1, TL, false, AFy, ctrl);
+      } else {  // upper border of the y axis (c == m1 == ny)
+        l = border_face2(gg, 3, 1, TyU, true, AFy, ctrl);
+      }
+      lam_max = (l < lam_max) ? lam_max : l;
+      // row c-1 is complete
+      if (c - 1 >= m0) {
+        const int r = c - 1;
+        const bool wrap_y = g.periodic[1] && r == ny - 1;
+        const uint32_t idx = col + (uint32_t)r * (uint32_t)nx;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          const double rhs = wrap_y ? (px[k] - AFy[k]) + pym[k] : (px[k] + pym[k]) - AFy[k];
+          out[k * fs + idx] = rk_update<STAGE>(gg, (STAGE == 1) ? 0.0 : Uc1[k], Xa[k], rhs, dt);
+        }
+      }
+    }
+    // x faces of row c
+    if (c >= m0 && c < m1) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) srow[k][lane] = Xb[k];
+      __syncthreads();
+      double FxL[NF], FxU[NF];
+      {
+        double sxl[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          const double xl = (lane > 0) ? srow[k][lane - 1] : 0.0;
+          const double xr = (lane < W2O - 1) ? srow[k][lane + 1] : 0.0;
+          const double dm = exl ? Xb[k] - xl : 0.0;
+          const double dp = exr ? xr - Xb[k] : 0.0;
+          sxl[k] = minmod(dp, dm);
+        }
+        recon_side(Xb, sxl, false, FxL);
+        recon_side(Xb, sxl, true, FxU);
+      }
+      Tup TxL;
+      make_tup(FxL, TxL);
+      sfl[0][lane] = TxL.c[0];
+      sfl[1][lane] = TxL.c[1];
+      sfl[2][lane] = TxL.c[2];
+      sfl[3][lane] = TxL.c[3];
+      sfl[4][lane] = TxL.c[4];
+      sfl[5][lane] = TxL.vel[0];
+      sfl[6][lane] = TxL.vel[1];
+      sfl[7][lane] = TxL.vel[2];
+      sfl[8][lane] = TxL.p;
+      sfl[9][lane] = TxL.a;
+      sfl[10][lane] = TxL.H;
+      __syncthreads();
+      double AFx[NF];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) AFx[k] = 0.0;
+      if (owner) {
+        Tup TR;
+        if (lane < W2O - 1) {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) TR.c[k] = sfl[k][lane + 1];
+          TR.vel[0] = sfl[5][lane + 1];
+          TR.vel[1] = sfl[6][lane + 1];
+          TR.vel[2] = sfl[7][lane + 1];
+          TR.p = sfl[8][lane + 1];
+          TR.a = sfl[9][lane + 1];
+          TR.H = sfl[10][lane + 1];
+        }
+        double l;
+        if (ex && exr) {
+          Tup TU;
+          make_tup(FxU, TU);
+          l = face_flux_area(TU, TR, 0, P2, A, halfA, AFx);
+          if (tup_negative(TU) | tup_negative(TR)) count_neg(TU, TR, ctrl);
+        } else if (exr) {  // this column is outside a non-periodic x border
+          l = border_face2(gg, 0, 0, TR, false, AFx, ctrl);
+        } else {  // the next column is outside
+          Tup TU;
+          make_tup(FxU, TU);
+          l = border_face2(gg, 1, 0, TU, true, AFx, ctrl);
+        }
+        lam_max = (l < lam_max) ? lam_max : l;
+      }
+#pragma unroll
+      for (int k = 0; k < NF; ++k) saf[k][lane] = AFx[k];
+      __syncthreads();
+      if (interior) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          const double am = saf[k][lane - 1];
+          px[k] = wrap_x ? (0.0 - AFx[k]) + am : (0.0 + am) - AFx[k];
+          pym[k] = AFy[k];
+        }
+      }
+    }
+    // keep row c's upper y face value, slide the window
+    if (eb) make_tup(FyU, TyU);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      Xa[k] = Xb[k];
+      Xb[k] = Xc[k];
+    }
+    ea = eb;
+    eb = ec;
+    ec = epf;
+    if (epf) pf.state(gg, dt, Xc);
+    {
+      bool e;
+      const int r = row_of(c + 3, e);
+      epf = e && ex && (c + 3 <= last);
+      if (epf) pf.issue(in, col + (uint32_t)r * (uint32_t)nx);
+    }
+    __syncthreads();
+  }
+  const unsigned long long b = wave_max_bits(lam_max);
+  if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[(mode == MODE_RHS_ONLY) ? 3 : STAGE - 1], b);
+}
+
+template <int STAGE, bool P2>
+static hipError_t launch2d_o2_t(const GridDev &g, int chunk, int mode, const double *U, const double *R,
+                                const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  const double *X = (STAGE == 3) ? W : U;
+  double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
+  const int ntx = (int)((g.n[0] + INNER2 - 1) / INNER2);
+  const int nch = (int)((g.n[1] + chunk - 1) / chunk);
+  hipLaunchKernelGGL((k_march2d_o2<STAGE, P2>), dim3((unsigned)(ntx * nch)), dim3(W2O), 0, s, g, X, U, R, out, ctrl,
+                     mode, chunk, ntx);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_march2d_o2(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
+                                   const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  const bool p2 = g.pow2A && g.pow2V;
+  switch (stage) {
+    case 1: return p2 ? launch2d_o2_t<1, true>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s)
+                      : launch2d_o2_t<1, false>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return p2 ? launch2d_o2_t<2, true>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s)
+                      : launch2d_o2_t<2, false>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return p2 ? launch2d_o2_t<3, true>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s)
+                       : launch2d_o2_t<3, false>(g, chunk, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
+}
+
 // ---- persistent batch ---------------------------------------------------------
 template <int STAGE>
 __global__ __launch_bounds__(W2) void k_batch2d(const BatchCase *cases, const int *items, int nitems,

@@ -93,3 +93,37 @@ def test_march_positivity_counter(mmf, orc):
         s.set_state(U)
         s.compute_rhs()
         assert s.positivity_counters() == o.counters(), kind
+
+
+O2_CASES = [
+    ("2d o2 free", (64, 64), [0] * 4),
+    ("2d o2 periodic ragged", (121, 37), [4, 4, 4, 4]),
+    ("2d o2 mixed bcs", (61, 60), [3, 1, 2, 0]),
+    ("2d o2 periodic x only", (60, 9), [4, 4, 1, 3]),
+    ("2d o2 thin", (3, 70), [4, 4, 0, 1]),
+    ("2d o2 one row", (130, 1), [0, 2, 1, 3]),
+]
+
+
+@pytest.mark.parametrize("name,n,bc", O2_CASES, ids=[c[0] for c in O2_CASES])
+def test_march_order2_2d_equals_gather(mmf, orc, name, n, bc):
+    """The 2D order-2 strip kernel (limited linear, each face once) against the
+    per-cell gather kernel: bit-identical RHS, steps, time control and
+    positivity counts for every chunk shape; and the oracle on the RHS."""
+    info = np.zeros((6, 5))
+    info[:] = [1.0, 0.3, -0.1, 0.0, 1.0]
+    s, U = build(mmf, 2, list(n), bc, info, None, order=2, seed=5)
+    ref = run_variant(mmf, s, U, mmf.KERNEL_GATHER)
+    for chunk in (16, 5, 1, 2, 1000):
+        got = run_variant(mmf, s, U, mmf.KERNEL_MARCH, 1, chunk)
+        assert s.kernel()[0] == mmf.KERNEL_MARCH
+        assert np.array_equal(got[0], ref[0]), (name, chunk, "rhs")
+        assert got[1] == ref[1], (name, chunk, "maxEig")
+        assert np.array_equal(got[2], ref[2]), (name, chunk, "state after 3 steps")
+        assert got[3] == ref[3], (name, chunk, "time control")
+        assert got[4] == ref[4], (name, chunk, "positivity counters")
+    po = orc.problem(orc.RADSOD, 2)
+    o = orc.Solver(po, 2, list(n), [-1.0, -1.0, -1.0], 2.0 / max(n), side_bc=bc + [0, 0], side_info=info)
+    Ro, mo = o.compute_rhs(U)
+    assert np.max(np.abs(ref[0] - Ro)) <= 1e-12 * np.max(np.abs(Ro))
+    assert ref[1] == mo
