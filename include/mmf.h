@@ -260,7 +260,7 @@ int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chun
  * reference: the face values of the limited linear polynomial
  * (reconstruction.cpp:67-97) are replaced by the output of a small network
  * (5x5 stencil of rho, rho u, rho v, rho E, normalised per cell -> 64 -> 64
- * (GELU) -> 16 outputs z), run in bf16 on the matrix cores with fp32
+ * (GELU, tanh form) -> 16 outputs z), run in bf16 on the matrix cores with fp32
  * accumulation inside the reconstruction kernel.  The network is a learned
  * limiter: per face and component the value is P + sigmoid(z) (Q - P), P the
  * minmod face value and Q the fifth-order upwind-biased linear value along

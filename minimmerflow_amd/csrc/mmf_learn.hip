@@ -22,7 +22,7 @@
 //                    non-periodic border, when the shock sensor fires, or
 //                    when a learned face state has non-positive density or
 //                    pressure.  Face states go to a [face][component] SoA.
-//   k_flux_fs        one thread per cell: the four faces from the face states
+//   k_flux_fs        four lanes per cell, one face each, from the face states
 //                    (border states from the BC), Rusanov flux, residual in
 //                    the reference face order, fused RK stage update.  With
 //                    every cell on the fallback this is bit-identical to the
@@ -53,17 +53,40 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // conservative component of stencil variable v (rho, rho u, rho v, rho E)
 __device__ __forceinline__ int var_of(int v) { return v == 3 ? FID_RHO_E : v; }
 
-__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
-
-__device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+// logistic function 1 / (1 + e^-z) from the hardware exp2 and reciprocal
+// (v_exp_f32, v_rcp_f32: about 1 ulp each); z -> -inf gives 0, +inf gives 1
+__device__ __forceinline__ float sigm_f(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
 }
 
-// fp32 -> bf16, round to nearest even
-__device__ __forceinline__ short f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+// GELU in its tanh form, 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+// = z sigmoid(v), v = 2 sqrt(2/pi) (z + 0.044715 z^3) (Hendrycks & Gimpel,
+// "Gaussian Error Linear Units", 2016); within 4.8e-4 of the erf form.
+constexpr float GELU_K = 1.5957691216057308f, GELU_C = 0.044715f;
+__device__ __forceinline__ float gelu_f(float z) { return z * sigm_f(GELU_K * (z + GELU_C * (z * z * z))); }
+
+__device__ __forceinline__ float gelu_d(float z) {
+  const float sg = sigm_f(GELU_K * (z + GELU_C * (z * z * z)));
+  return sg + z * (sg * (1.0f - sg)) * (GELU_K * (1.0f + 3.0f * GELU_C * (z * z)));
+}
+
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
+
+// four fp32 values -> one bf16 fragment (two packed conversions)
+__device__ __forceinline__ s4 bf4(float a, float b, float c, float d) {
+  const f2v lo = {a, b}, hi = {c, d};
+  const uint32_t l = __builtin_bit_cast(uint32_t, __builtin_convertvector(lo, bf2));
+  const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(hi, bf2));
+  s4 r;
+  r[0] = (short)(l & 0xFFFFu);
+  r[1] = (short)(l >> 16);
+  r[2] = (short)(h & 0xFFFFu);
+  r[3] = (short)(h >> 16);
+  return r;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -86,8 +109,8 @@ __device__ __forceinline__ f4 mlp16(const s4 *F, const float *B, const s4 *xb, i
 #pragma unroll
     for (int kb = 0; kb < 7; ++kb)
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(mt * 7 + kb) * 64 + lane], xb[kb], acc, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) h1[mt][i] = f2bf(gelu_f(acc[i] + B[16 * mt + 4 * g + i]));
+    const float *b = B + 16 * mt + 4 * g;
+    h1[mt] = bf4(gelu_f(acc[0] + b[0]), gelu_f(acc[1] + b[1]), gelu_f(acc[2] + b[2]), gelu_f(acc[3] + b[3]));
   }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
@@ -95,8 +118,8 @@ __device__ __forceinline__ f4 mlp16(const s4 *F, const float *B, const s4 *xb, i
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb)
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(28 + mt * 4 + kb) * 64 + lane], h1[kb], acc, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) h2[mt][i] = f2bf(gelu_f(acc[i] + B[NH + 16 * mt + 4 * g + i]));
+    const float *b = B + NH + 16 * mt + 4 * g;
+    h2[mt] = bf4(gelu_f(acc[0] + b[0]), gelu_f(acc[1] + b[1]), gelu_f(acc[2] + b[2]), gelu_f(acc[3] + b[3]));
   }
   f4 y = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -132,7 +155,12 @@ __device__ __forceinline__ void candidates_norm(const float *x, int o, float &P,
 
 // ---- weight packing ------------------------------------------------------------
 // Fragment fi (layer 1: mt*7+kb; layer 2: 28+mt*4+kb; layer 3: 44+kb), lane l,
-// element i holds W[16 mt + l%16][16 kb + 4 (l/16) + i] (zero past K).
+// element i holds W[16 mt + l%16][k] with, for layers 2 and 3, k = 16 kb +
+// 4 (l/16) + i.  Layer 1 reads its inputs in a permuted order: lane group
+// g = l/16 carries stencil variable g, stencil position r = 4 kb + i (zero
+// for r >= 25), i.e. input k = 25 g + r of the stored row layout.  The B
+// operand built by the kernels follows the same order, so the product is the
+// network's; each lane then only touches its own variable's stencil.
 __global__ __launch_bounds__(64) void k_learn_pack(const float *P, short *frags) {
   const int fi = blockIdx.x, lane = threadIdx.x;
   int off, K, mt, kb;
@@ -146,8 +174,14 @@ __global__ __launch_bounds__(64) void k_learn_pack(const float *P, short *frags)
   const int m = 16 * mt + (lane & 15);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int k = 16 * kb + 4 * (lane >> 4) + i;
-    const float w = (k < K) ? P[off + m * K + k] : 0.0f;
+    float w;
+    if (fi < 28) {
+      const int r = 4 * kb + i;
+      w = (r < 25) ? P[off + m * K + 25 * (lane >> 4) + r] : 0.0f;
+    } else {
+      const int k = 16 * kb + 4 * (lane >> 4) + i;
+      w = (k < K) ? P[off + m * K + k] : 0.0f;
+    }
     frags[(fi * 64 + lane) * 4 + i] = f2bf(w);
   }
 }
@@ -172,8 +206,8 @@ __global__ __launch_bounds__(64) void k_learn_forward_mfma(const float *P, const
   for (int kb = 0; kb < 7; ++kb)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int k = 16 * kb + 4 * g + i;
-      xb[kb][i] = f2bf((live && k < NIN) ? X[row * NIN + k] : 0.0f);
+      const int r = 4 * kb + i;  // stencil position of variable g
+      xb[kb][i] = f2bf((live && r < 25) ? X[row * NIN + 25 * g + r] : 0.0f);
     }
   const f4 y = mlp16(F, sB, xb, lane);
   if (live) {
@@ -289,7 +323,7 @@ __global__ __launch_bounds__(256) void k_learn_train(const float *P, const float
     // model output: P + sigmoid(z) (Q - P)
     float Pn, Qn;
     candidates_norm(&sx[s][0], o, Pn, Qn);
-    const float sg = 1.0f / (1.0f + expf(-z));
+    const float sg = sigm_f(z);
     const float r = (Pn + sg * (Qn - Pn)) - st[s][o];
     lsum += sw[s] * r * r;
     sdy[s][o] = scale * sw[s] * r * (sg * (1.0f - sg)) * (Qn - Pn);
@@ -588,8 +622,6 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   __shared__ uint8_t rex[RGY][RGX];
   __shared__ s4 sF[NFRAG * 64];
   __shared__ float sB[2 * NH + NOUT];
-  __shared__ double sc[RTY][16][4];  // centre value per cell and stencil variable
-  __shared__ float sis[RTY][16][4];   // 1 / scale
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
   const double dt = (STAGE == 2) ? step_dt(ctrl) : 0.0;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
@@ -626,19 +658,22 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   const bool learnable = live && (g.periodic[0] || (gx >= 2 && gx <= nx - 3)) &&
                          (g.periodic[1] || (gy >= 2 && gy <= ny - 3));
   bool fires = false;
+  // this lane's stencil variable (lane group grp), its centre and 1 / scale
+  const int kv = var_of(grp);
+  double cv = 0.0;
+  float inv_s = 0.0f;
   if (use_net) {
-    // centre and scale of stencil variable grp of this lane's cell
-    const int kv = var_of(grp);
-    const double cv = reg[kv][ry][rx];
+    cv = reg[kv][ry][rx];
     double dev = 0.0;
+#pragma unroll
     for (int dj = -2; dj <= 2; ++dj)
+#pragma unroll
       for (int di = -2; di <= 2; ++di) {
         const double d = fabs(reg[kv][ry + dj][rx + di] - cv);
         dev = (d > dev) ? d : dev;
       }
     const double tau = TAU_REL * (fabs(reg[FID_RHO][ry][rx]) + fabs(reg[FID_RHO_E][ry][rx]));
-    sc[w][c][grp] = cv;
-    sis[w][c][grp] = (float)(1.0 / ((dev >= tau) ? dev : tau));
+    inv_s = (float)(1.0 / ((dev >= tau) ? dev : tau));
     // shock sensor: lane group 0 on rho, group 1 on p; fires where
     // |q+ - 2 q0 + q-| > theta (q+ + 2 q0 + q-) along either axis
     if (grp < 2) {
@@ -661,7 +696,6 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
       }
     }
   }
-  __syncthreads();
 
   // plain limited linear face value of face grp (reconstruction.cpp:67-97)
   auto plain_face = [&](double *P) {
@@ -692,18 +726,16 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   if (use_net && wave_live) {
     s4 xb[7];
 #pragma unroll
-    for (int kb = 0; kb < 7; ++kb)
+    for (int kb = 0; kb < 7; ++kb) {
+      float xv[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int k = 16 * kb + 4 * grp + i;
-        float xv = 0.0f;
-        if (live && k < NIN) {
-          const int v = k / 25, rr = k % 25;
-          const double S = reg[var_of(v)][ry - 2 + rr / 5][rx - 2 + rr % 5];
-          xv = (float)(S - sc[w][c][v]) * sis[w][c][v];
-        }
-        xb[kb][i] = f2bf(xv);
+        const int r = 4 * kb + i;  // stencil position, compile-time after unrolling
+        xv[i] = 0.0f;
+        if (r < 25 && live) xv[i] = (float)(reg[kv][ry - 2 + r / 5][rx - 2 + r % 5] - cv) * inv_s;
       }
+      xb[kb] = bf4(xv[0], xv[1], xv[2], xv[3]);
+    }
     const f4 y = mlp16(sF, sB, xb, lane);
     // face neighbour: the cell across face grp
     const int a = grp >> 1, side = grp & 1;
@@ -721,7 +753,7 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
       }
       const double Pm = u[2] + 0.5 * minmod(u[3] - u[2], u[2] - u[1]);
       const double Q5 = ((((2.0 * u[0] - 13.0 * u[1]) + 47.0 * u[2]) + 27.0 * u[3]) - 3.0 * u[4]) * (1.0 / 60.0);
-      const double sg = (double)(1.0f / (1.0f + expf(-y[v])));
+      const double sg = (double)sigm_f(y[v]);
       double f = Pm + sg * (Q5 - Pm);
       if (run.clip) {
         // between the means of the two cells sharing the face
@@ -753,80 +785,121 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
 #pragma unroll
     for (int k = 0; k < NF; ++k) run.fs[(grp * NF + k) * fs + cell] = Fv[k];
   }
-  const unsigned long long lm = __ballot(live && learned && grp == 0);
-  const unsigned long long fm = __ballot(live && !learned && grp == 0);
-  if (lane == 0 && STAGE == 1) {
-    if (lm) atomicAdd(&run.count[0], (unsigned long long)__popcll(lm));
-    if (fm) atomicAdd(&run.count[1], (unsigned long long)__popcll(fm));
+  if (STAGE == 1) {
+    // per-block counts: one plain update of this block's own slots (no
+    // contended atomics; launches on the stream are ordered)
+    __shared__ int scount[2][4];
+    const unsigned long long lm = __ballot(live && learned && grp == 0);
+    const unsigned long long fm = __ballot(live && !learned && grp == 0);
+    if (lane == 0) {
+      scount[0][w] = __popcll(lm);
+      scount[1][w] = __popcll(fm);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int64_t b = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+      run.count[2 * b] += (unsigned long long)(scount[0][0] + scount[0][1] + scount[0][2] + scount[0][3]);
+      run.count[2 * b + 1] += (unsigned long long)(scount[1][0] + scount[1][1] + scount[1][2] + scount[1][3]);
+    }
   }
 }
 
+// One face of a cell along axis A (lower or upper per lane) from the face
+// states: border state from the BC where the neighbour is outside, Rusanov
+// flux, the area-weighted flux into LDS.  Returns the face eigenvalue.
+template <int A>
+__device__ __forceinline__ double fs_face(const GridDev &g, const double *FS, const int64_t *q, int64_t idx,
+                                          bool upper, Ctrl *ctrl, double *af, int stride) {
+  const int64_t fs = g.fs;
+  const int f = 2 * A + (upper ? 1 : 0);
+  int64_t nb = 0;
+  const bool en = neighbour(g, q, A, upper ? +1 : -1, nb);
+  double own[NF], oth[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) own[k] = FS[(f * NF + k) * fs + idx];
+  if (en) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) oth[k] = FS[((f ^ 1) * NF + k) * fs + nb];
+  } else if (upper) {
+    border_state(g, 2 * A + 1, A, own, oth);
+  } else {
+    border_state(g, 2 * A, A, own, oth);
+  }
+  double cl[NF], cr[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    cl[k] = upper ? own[k] : oth[k];
+    cr[k] = upper ? oth[k] : own[k];
+  }
+  Tup TL, TR;
+  make_tup(cl, TL);
+  make_tup(cr, TR);
+  double F[NF];
+  const double lam = face_flux(TL, TR, A, F);
+  // each face counted once: a cell's lower face only at a border
+  if (upper || !en) count_neg(TL, TR, ctrl);
+  const double Ar = g.A;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) af[k * stride] = Ar * F[k];
+  return lam;
+}
+
+// 64 cells per block of four waves.  Wave w takes axis w&1 for the 32 cells
+// of half w>>1, lanes 0-31 their lower faces and lanes 32-63 their upper
+// faces, so the axis is uniform across a wave.  The area-weighted fluxes go
+// through LDS; then thread t sums component t>>6 (threads 0-63 also
+// component 4) of cell t&63 in the reference face order and writes that
+// component's RK stage update.
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_flux_fs(GridDev g, StageIn<STAGE> in, const double *FS, const double *U,
                                                  double *out, Ctrl *ctrl, int mode) {
+  __shared__ double saf[4][NF][64];  // [face][component][cell of the block]
   const int slot = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;
   if (STAGE >= 2) in.dt = step_dt(ctrl);
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int axis = wv & 1;
+  const bool upper = lane >= 32;
+  const int cb = 32 * (wv >> 1) + (lane & 31);
+  const int64_t base = (int64_t)blockIdx.x * 64;
   double lam_max = 0.0;
+  {
+    const int64_t idx = base + cb;
+    if (idx < g.N) {
+      const int64_t q[3] = {idx % g.n[0], idx / g.n[0], 0};
+      double *af = &saf[2 * axis + (upper ? 1 : 0)][0][cb];
+      lam_max = (axis == 0) ? fs_face<0>(g, FS, q, idx, upper, ctrl, af, 64)
+                            : fs_face<1>(g, FS, q, idx, upper, ctrl, af, 64);
+    }
+  }
+  __syncthreads();
+  const int c = threadIdx.x & 63;
+  const int64_t idx = base + c;
   if (idx < g.N) {
     const int64_t fs = g.fs;
-    const int64_t q[3] = {idx % g.n[0], idx / g.n[0], 0};
-    double Xc[NF];
-    in.load(g, idx, Xc);
-    double rhs[NF] = {0., 0., 0., 0., 0.};
+    const bool wrap_x = g.periodic[0] && (idx % g.n[0]) == g.n[0] - 1;
+    const bool wrap_y = g.periodic[1] && (idx / g.n[0]) == g.n[1] - 1;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      int64_t nm1 = 0, np1 = 0;
-      const bool em1 = neighbour(g, q, a, -1, nm1);
-      const bool ep1 = neighbour(g, q, a, +1, np1);
-      double Lm[NF], Rm[NF], Lp[NF], Rp[NF];
-#pragma unroll
-      for (int k = 0; k < NF; ++k) {
-        Rm[k] = FS[((2 * a) * NF + k) * fs + idx];
-        Lp[k] = FS[((2 * a + 1) * NF + k) * fs + idx];
-      }
-      if (em1) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Lm[k] = FS[((2 * a + 1) * NF + k) * fs + nm1];
-      } else {
-        border_state(g, 2 * a, a, Rm, Lm);
-      }
-      if (ep1) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Rp[k] = FS[((2 * a) * NF + k) * fs + np1];
-      } else {
-        border_state(g, 2 * a + 1, a, Lp, Rp);
-      }
-      Tup TLm, TRm, TLp, TRp;
-      make_tup(Rm, TRm);
-      make_tup(Lp, TLp);
-      make_tup(Lm, TLm);
-      make_tup(Rp, TRp);
-      double Fm[NF], Fp[NF];
-      const double lm = face_flux(TLm, TRm, a, Fm);
-      const double lp = face_flux(TLp, TRp, a, Fp);
-      lam_max = (lm < lam_max) ? lam_max : lm;
-      lam_max = (lp < lam_max) ? lam_max : lp;
-      if (!em1) count_neg(TLm, TRm, ctrl);
-      count_neg(TLp, TRp, ctrl);
-      const double A = g.A;
-      const bool wrap_first = g.periodic[a] && (q[a] == g.n[a] - 1);
-      if (wrap_first) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) { rhs[k] -= A * Fp[k]; rhs[k] += A * Fm[k]; }
-      } else {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) { rhs[k] += A * Fm[k]; rhs[k] -= A * Fp[k]; }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < NF; ++k) {
+    for (int r = 0; r < 2; ++r) {
+      if (r == 1 && wv != 0) break;
+      const int k = (r == 0) ? wv : 4;
+      double rhs = 0.0;
+      if (wrap_x) { rhs -= saf[1][k][c]; rhs += saf[0][k][c]; }
+      else { rhs += saf[0][k][c]; rhs -= saf[1][k][c]; }
+      if (wrap_y) { rhs -= saf[3][k][c]; rhs += saf[2][k][c]; }
+      else { rhs += saf[2][k][c]; rhs -= saf[3][k][c]; }
+      // the stage input of component k (main.cpp:421 for stage 2)
+      const double x = (STAGE == 2) ? in.U[k * in.fs + idx] + div_vol(g, in.dt * in.R[k * in.fs + idx])
+                                    : in.X[k * in.fs + idx];
       const double u = (STAGE == 1) ? 0.0 : U[k * fs + idx];
-      out[k * fs + idx] = rk_update<STAGE>(g, u, Xc[k], rhs[k], in.dt);
+      out[k * fs + idx] = rk_update<STAGE>(g, u, x, rhs, in.dt);
     }
   }
   block_max_to<256>(&ctrl->eig[slot], lam_max, threadIdx.x);
+}
+
+int64_t learn_count_slots(const GridDev &g) {
+  return 2 * ((g.n[0] + RTX - 1) / RTX) * ((g.n[1] + RTY - 1) / RTY);
 }
 
 template <int STAGE>
@@ -857,7 +930,7 @@ static hipError_t flux_t(const GridDev &g, const LearnRun &run, int mode, const 
   in.R = R;
   in.X = (STAGE == 3) ? W : U;
   double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
-  hipLaunchKernelGGL((k_flux_fs<STAGE>), dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, in, run.fs, U, out,
+  hipLaunchKernelGGL((k_flux_fs<STAGE>), dim3((unsigned)((g.N + 63) / 64)), dim3(256), 0, s, g, in, run.fs, U, out,
                      ctrl, mode);
   return hipGetLastError();
 }

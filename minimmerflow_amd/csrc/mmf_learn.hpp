@@ -46,8 +46,13 @@ struct LearnRun {
   int clip;                   // clip learned face values between the means of the two cells of the face
   double blend;               // face value = plain + blend (learned - plain)
   double *fs;                 // face states, [4 faces][5 comps][fs]
-  unsigned long long *count;  // [0] cells reconstructed by the network, [1] cells on the fallback
+  // per reconstruction block b: [2b] cells reconstructed by the network,
+  // [2b+1] cells on the fallback (learn_count_slots(g) entries)
+  unsigned long long *count;
 };
+
+// Number of counter entries the reconstruction kernel uses on grid g.
+int64_t learn_count_slots(const GridDev &g);
 
 hipError_t launch_learn_pack(const float *params, short *frags, hipStream_t s);
 hipError_t launch_learn_train(const float *params, const float *X, const float *T, const float *W, int64_t rows,

@@ -323,8 +323,9 @@ int mmf_set_learned_reconstruction(mmf_ctx *ctx, const mmf_learn *n, double sens
   CCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (!ctx->dFS) CCHK(ctx, hipMalloc(&ctx->dFS, sizeof(double) * 4 * 5 * (size_t)ctx->g.fs));
   if (!ctx->dlcount) {
-    CCHK(ctx, hipMalloc(&ctx->dlcount, sizeof(unsigned long long) * 2));
-    CCHK(ctx, hipMemset(ctx->dlcount, 0, sizeof(unsigned long long) * 2));
+    const size_t bytes = sizeof(unsigned long long) * (size_t)mmf::learn_count_slots(ctx->g);
+    CCHK(ctx, hipMalloc(&ctx->dlcount, bytes));
+    CCHK(ctx, hipMemset(ctx->dlcount, 0, bytes));
   }
   if (n) CCHK(ctx, hipStreamSynchronize(n->stream));
   ctx->learn.net.frags = n ? n->dF : nullptr;
@@ -356,8 +357,14 @@ int mmf_get_learned_counts(mmf_ctx *ctx, int64_t *learned_cells, int64_t *fallba
   if (!ctx) return MMF_ERR_INVALID;
   unsigned long long h[2] = {0, 0};
   if (ctx->dlcount) {
+    std::vector<unsigned long long> slots((size_t)mmf::learn_count_slots(ctx->g));
     CCHK(ctx, hipStreamSynchronize(ctx->stream));
-    CCHK(ctx, hipMemcpy(h, ctx->dlcount, sizeof(h), hipMemcpyDeviceToHost));
+    CCHK(ctx, hipMemcpy(slots.data(), ctx->dlcount, sizeof(unsigned long long) * slots.size(),
+                        hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < slots.size(); i += 2) {
+      h[0] += slots[i];
+      h[1] += slots[i + 1];
+    }
   }
   if (learned_cells) *learned_cells = (int64_t)h[0];
   if (fallback_cells) *fallback_cells = (int64_t)h[1];
@@ -368,7 +375,7 @@ int mmf_reset_learned_counts(mmf_ctx *ctx) {
   if (!ctx) return MMF_ERR_INVALID;
   if (ctx->dlcount) {
     CCHK(ctx, hipStreamSynchronize(ctx->stream));
-    CCHK(ctx, hipMemset(ctx->dlcount, 0, sizeof(unsigned long long) * 2));
+    CCHK(ctx, hipMemset(ctx->dlcount, 0, sizeof(unsigned long long) * (size_t)mmf::learn_count_slots(ctx->g)));
   }
   return MMF_OK;
 }

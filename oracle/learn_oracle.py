@@ -22,7 +22,8 @@ Definitions shared with the HIP kernels (every choice is this build's own):
     P the minmod value and Q the fifth-order upwind-biased linear value along
     the face's axis (a learned limiter); rho w is copied from the cell;
     training targets are (F - c) / s;
-  * network 100 -> 64 -> 64 -> 16, GELU (erf form) on both hidden layers;
+  * network 100 -> 64 -> 64 -> 16, GELU on both hidden layers in its tanh
+    form z sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)) (Hendrycks & Gimpel 2016);
   * loss sum_s w_s sum_o (yhat - t)^2 / (16 B) over a batch of B rows, with
     yhat = P + sigmoid(z) (Q - P) in normalised units;
   * Adam (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected);
@@ -36,7 +37,6 @@ Definitions shared with the HIP kernels (every choice is this build's own):
     over the fine faces that tile it (fine run restricted by the factor r).
 """
 import numpy as np
-from scipy.special import erf
 
 GAMMA = 1.4
 NIN, NH, NOUT = 100, 64, 16
@@ -44,8 +44,8 @@ LAYOUT = [("W1", (NH, NIN)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)), ("W
 NPARAMS = sum(int(np.prod(s)) for _, s in LAYOUT)
 VARS = (0, 1, 2, 4)
 TAU_REL = 1e-3  # floor of the per-cell scale, relative to |rho| + |rho E|
-SQRT2 = np.sqrt(2.0)
-INV_SQRT_2PI = 1.0 / np.sqrt(2.0 * np.pi)
+GELU_K = 2.0 * np.sqrt(2.0 / np.pi)
+GELU_C = 0.044715
 
 
 # ---- network ---------------------------------------------------------------------
@@ -65,11 +65,12 @@ def pack(d):
 
 
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / SQRT2))
+    return z * sigmoid(GELU_K * (z + GELU_C * z ** 3))
 
 
 def gelu_d(z):
-    return 0.5 * (1.0 + erf(z / SQRT2)) + z * np.exp(-0.5 * z * z) * INV_SQRT_2PI
+    sg = sigmoid(GELU_K * (z + GELU_C * z ** 3))
+    return sg + z * sg * (1.0 - sg) * GELU_K * (1.0 + 3.0 * GELU_C * z * z)
 
 
 def forward(p, X):

@@ -27,12 +27,20 @@ def test_layout_size():
 
 
 def test_gelu_closed_forms():
+    from scipy.special import erf
     z = np.linspace(-4, 4, 41)
     assert LO.gelu(np.array(0.0)) == 0.0
     assert LO.gelu_d(np.array(0.0)) == 0.5
-    assert np.allclose(LO.gelu(z) - LO.gelu(-z), z, rtol=0, atol=1e-15)  # Phi(z) + Phi(-z) = 1
+    # z sigmoid(v(z)) with v odd: sigmoid(v) + sigmoid(-v) = 1
+    assert np.allclose(LO.gelu(z) - LO.gelu(-z), z, rtol=0, atol=1e-15)
     h = 1e-6
     assert np.allclose(LO.gelu_d(z), (LO.gelu(z + h) - LO.gelu(z - h)) / (2 * h), atol=1e-8)
+    # the tanh form stays within 4.8e-4 of the exact z Phi(z) (a dropped cubic
+    # term or a wrong factor in v is off by 1e-2 or more)
+    w = np.linspace(-10, 10, 20001)
+    exact = 0.5 * w * (1.0 + erf(w / np.sqrt(2.0)))
+    err = np.max(np.abs(LO.gelu(w) - exact))
+    assert 4e-4 < err < 4.8e-4
 
 
 def test_backward_matches_finite_differences():
