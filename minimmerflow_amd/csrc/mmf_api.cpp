@@ -86,12 +86,16 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
   }
   hipError_t e;
-  if (ctx->learn_on) {
-    // face states (learned or plain), then flux + RK stage update from them
+  if (ctx->learn_on && ctx->kernel == mmf::KERNEL_GATHER) {
+    // face states (learned or plain) through HBM, then flux + RK stage update
     e = mmf::launch_recon_learned(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dctrl, ctx->stream);
     if (e == hipSuccess)
       e = mmf::launch_flux_fs(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
                               ctx->dctrl, ctx->stream);
+  } else if (ctx->learn_on) {
+    // the same in one launch per stage (face states in LDS)
+    e = mmf::launch_stage_learned(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW,
+                                  ctx->dU, ctx->dctrl, ctx->stream);
   } else {
     const bool march = use_march(ctx);
     e = march ? mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW,
@@ -418,10 +422,7 @@ int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
   int r2 = read_ctrl(ctx);
   if (r2 != MMF_OK) return r2;
   if (max_eig) {
-    unsigned long long b = ctx->hctrl->eig[3];
-    double v;
-    std::memcpy(&v, &b, sizeof(v));
-    *max_eig = v;
+    *max_eig = mmf::ctrl_stage_max(*ctx->hctrl, 3);
   }
   return MMF_OK;
 }
@@ -454,6 +455,8 @@ int enqueue_steps(mmf_ctx *ctx, int64_t nsteps) {
       int rc = launch_stage_timed(ctx, stage, mmf::MODE_STEP);
       if (rc != MMF_OK) return rc;
     }
+    // the fused learned stage kernel advances the time itself
+    if (ctx->learn_on && ctx->kernel != mmf::KERNEL_GATHER) continue;
     hipError_t e = mmf::launch_advance(ctx->dctrl, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "advance kernel");
   }

@@ -36,15 +36,36 @@ struct MarchShape {
   int chunk;  // planes per block along the march axis
 };
 
+// Stage maxima and block counters are spread over EIG_SPLIT cache lines each
+// (block b uses line b % EIG_SPLIT): device-scope atomics on one address
+// from hundreds of blocks serialise at the point of coherence (about 10 ns
+// each), on different lines they proceed in parallel.
+constexpr int EIG_SPLIT = 16;
+constexpr int LINE_U64 = 16;  // 128 bytes
+
 // Time-control block resident in device memory.
 struct Ctrl {
   double t, tmax, cfl, hmin;
   double dt;                     // dt of the last completed step
-  unsigned long long eig[4];     // running stage maxima (bits), slot 3: compute_rhs
   double last_eig[3];            // stage maxima of the last completed step
   long long step;
   unsigned long long neg_p, neg_rho;
+  // running stage maxima (bits) in entry [s][k][0], s = 3: compute_rhs; the
+  // stage maximum is the largest over k
+  alignas(128) unsigned long long eig[4][EIG_SPLIT][LINE_U64];
+  // blocks of a step-closing launch that have finished, per line
+  // ([k][0]) and lines completed ([EIG_SPLIT][0]); 0 between launches
+  unsigned long long done[EIG_SPLIT + 1][LINE_U64];
 };
+
+// Host view of stage maximum s (the largest entry over the lines).
+inline double ctrl_stage_max(const Ctrl &c, int s) {
+  unsigned long long m = 0;
+  for (int k = 0; k < EIG_SPLIT; ++k) m = (c.eig[s][k][0] > m) ? c.eig[s][k][0] : m;
+  double v;
+  __builtin_memcpy(&v, &m, sizeof(v));
+  return v;
+}
 
 enum StageMode { MODE_STEP = 0, MODE_RHS_ONLY = 1 };
 

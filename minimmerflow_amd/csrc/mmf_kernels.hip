@@ -187,27 +187,14 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
       }
     }
   }
-  block_max_to<BLOCK>(&ctrl->eig[slot], lam_max, threadIdx.x);
+  block_max_to<BLOCK>(ctrl, slot, lam_max, threadIdx.x);
 }
 
 // t += dt after stage 3 (main.cpp:505-506); resets the stage maxima.
-__global__ void k_advance(Ctrl *c) {
-  if (threadIdx.x != 0) return;
-  if (c->t < c->tmax) {
-    double dt = step_dt(c);
-    c->dt = dt;
-    c->t += dt;
-    c->step += 1;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) c->last_eig[s] = __longlong_as_double((long long)c->eig[s]);
-  }
-  c->eig[0] = 0ull;
-  c->eig[1] = 0ull;
-  c->eig[2] = 0ull;
-}
+__global__ void k_advance(Ctrl *c) { advance_ctrl_wave(c, threadIdx.x); }
 
 __global__ void k_zero_slot(Ctrl *c, int slot) {
-  if (threadIdx.x == 0) c->eig[slot] = 0ull;
+  if (threadIdx.x < EIG_SPLIT) c->eig[slot][threadIdx.x][0] = 0ull;
 }
 
 template <int DIM, int ORDER, int STAGE, bool MASK>

@@ -63,7 +63,12 @@ __device__ __forceinline__ float sigm_f(float z) {
 // = z sigmoid(v), v = 2 sqrt(2/pi) (z + 0.044715 z^3) (Hendrycks & Gimpel,
 // "Gaussian Error Linear Units", 2016); within 4.8e-4 of the erf form.
 constexpr float GELU_K = 1.5957691216057308f, GELU_C = 0.044715f;
-__device__ __forceinline__ float gelu_f(float z) { return z * sigm_f(GELU_K * (z + GELU_C * (z * z * z))); }
+__device__ __forceinline__ float gelu_f(float z) {
+  // z / (1 + 2^(z (a + b z^2))), a = -K log2(e), b = -K C log2(e)
+  const float e = __builtin_amdgcn_exp2f(z * fmaf(-GELU_K * GELU_C * 1.4426950408889634f, z * z,
+                                                   -GELU_K * 1.4426950408889634f));
+  return z * __builtin_amdgcn_rcpf(1.0f + e);
+}
 
 __device__ __forceinline__ float gelu_d(float z) {
   const float sg = sigm_f(GELU_K * (z + GELU_C * (z * z * z)));
@@ -612,51 +617,62 @@ hipError_t launch_learn_restrict(const GridDev &fine, const double *Uf, int fact
   return hipGetLastError();
 }
 
-// ---- reconstruction kernel ----------------------------------------------------------
-constexpr int RTX = 16, RTY = 4, RGX = RTX + 4, RGY = RTY + 4;
-
-template <int STAGE>
-__global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, const double *U, const double *R,
-                                                       const double *Xin, Ctrl *ctrl, int mode) {
-  __shared__ double reg[NF][RGY][RGX];
-  __shared__ uint8_t rex[RGY][RGX];
-  __shared__ s4 sF[NFRAG * 64];
-  __shared__ float sB[2 * NH + NOUT];
-  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
-  const double dt = (STAGE == 2) ? step_dt(ctrl) : 0.0;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
+// ---- reconstruction kernels ----------------------------------------------------------
+// Stage state of an RX x RY region whose first cell is (xo, yo), wrapped on
+// periodic axes; ex = 0 (and zero state) where a cell lies beyond a
+// non-periodic border.  Stage 2 rebuilds W1 = U + dt R / V (main.cpp:421).
+template <int STAGE, int RX, int RY, int NT>
+__device__ __forceinline__ void load_region(const GridDev &g, int xo, int yo, const double *U, const double *R,
+                                            const double *Xin, double dt, double (&reg)[NF][RY][RX],
+                                            uint8_t (&rex)[RY][RX], double (&pre)[RY][RX], int tid) {
   const int nx = (int)g.n[0], ny = (int)g.n[1];
   const int64_t fs = g.fs;
-  const int x0 = blockIdx.x * RTX, y0 = blockIdx.y * RTY;
-  const bool use_net = run.net.frags != nullptr;
-
-  for (int e = tid; e < RGX * RGY; e += 256) {
-    const int ry = e / RGX, rx = e % RGX;
-    int gx = x0 - 2 + rx, gy = y0 - 2 + ry;
+  for (int e = tid; e < RX * RY; e += NT) {
+    const int ry = e / RX, rx = e % RX;
+    int gx = xo + rx, gy = yo + ry;
     const bool ex = ((gx >= 0 && gx < nx) || g.periodic[0]) && ((gy >= 0 && gy < ny) || g.periodic[1]);
-    gx = ((gx % nx) + nx) % nx;
-    gy = ((gy % ny) + ny) % ny;
+    if (gx < 0 || gx >= nx) gx = ((gx % nx) + nx) % nx;
+    if (gy < 0 || gy >= ny) gy = ((gy % ny) + ny) % ny;
     rex[ry][rx] = ex;
     const int64_t i = gx + (int64_t)nx * gy;
+    double v[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k)
-      reg[k][ry][rx] = !ex ? 0.0 : (STAGE == 2) ? U[k * fs + i] + div_vol(g, dt * R[k * fs + i]) : Xin[k * fs + i];
+    for (int k = 0; k < NF; ++k) {
+      v[k] = !ex ? 0.0 : (STAGE == 2) ? U[k * fs + i] + div_vol(g, dt * R[k * fs + i]) : Xin[k * fs + i];
+      reg[k][ry][rx] = v[k];
+    }
+    // p = (gamma - 1) (rho E - |rho u|^2 / (2 rho)) for the shock sensor
+    const double m2 = (v[FID_RHO_U] * v[FID_RHO_U] + v[FID_RHO_V] * v[FID_RHO_V]) + v[FID_RHO_W] * v[FID_RHO_W];
+    pre[ry][rx] = ex ? (GAMMA - 1.0) * (v[FID_RHO_E] - 0.5 * m2 / v[FID_RHO]) : 0.0;
   }
-  if (use_net) {
-    const s4 *gF = (const s4 *)run.net.frags;
-    for (int e = tid; e < NFRAG * 64; e += 256) sF[e] = gF[e];
-    const float *P = run.net.params;
-    if (tid < 2 * NH + NOUT)
-      sB[tid] = (tid < NH) ? P[OFF_B1 + tid] : (tid < 2 * NH) ? P[OFF_B2 + tid - NH] : P[OFF_B3 + tid - 2 * NH];
-  }
-  __syncthreads();
+}
 
-  const int gx = x0 + c, gy = y0 + w;
-  const bool wave_live = gy < ny;
-  const bool live = wave_live && gx < nx;
-  const int rx = c + 2, ry = w + 2;
-  const bool learnable = live && (g.periodic[0] || (gx >= 2 && gx <= nx - 3)) &&
-                         (g.periodic[1] || (gy >= 2 && gy <= ny - 3));
+// Packed weight fragments and biases of the network into LDS.
+template <int NT>
+__device__ __forceinline__ void load_net(const LearnRun &run, s4 *sF, float *sB, int tid) {
+  if (run.net.frags == nullptr) return;
+  const s4 *gF = (const s4 *)run.net.frags;
+  for (int e = tid; e < NFRAG * 64; e += NT) sF[e] = gF[e];
+  const float *P = run.net.params;
+  for (int e = tid; e < 2 * NH + NOUT; e += NT)
+    sB[e] = (e < NH) ? P[OFF_B1 + e] : (e < 2 * NH) ? P[OFF_B2 + e - NH] : P[OFF_B3 + e - 2 * NH];
+}
+
+// Face grp = lane / 16 (0 x-, 1 x+, 2 y-, 3 y+) of the cell at region position
+// (rx, ry); lane % 16 is the cell's column of the network.  Fv receives the
+// learned value where the cell is learnable, its sensor is quiet and all
+// four learned face states are positive, else the plain limited linear value
+// (reconstruction.cpp:67-97).  The network runs when any lane of the wave is
+// live (wave_live, uniform over the wave).  Returns whether the cell's faces
+// are the learned ones.  Every cell's result depends only on its own
+// stencil, so any kernel that places the cell in some lane gets the same bits.
+template <int RX, int RY>
+__device__ __forceinline__ bool cell_face(const GridDev &g, const LearnRun &run, const double (&reg)[NF][RY][RX],
+                                          const uint8_t (&rex)[RY][RX], const double (&pre)[RY][RX], const s4 *sF,
+                                          const float *sB, int rx, int ry, bool live, bool learnable, bool wave_live,
+                                          int lane, double *Fv) {
+  const int c = lane & 15, grp = lane >> 4;
+  const bool use_net = run.net.frags != nullptr;
   bool fires = false;
   // this lane's stencil variable (lane group grp), its centre and 1 / scale
   const int kv = var_of(grp);
@@ -669,29 +685,21 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
     for (int dj = -2; dj <= 2; ++dj)
 #pragma unroll
       for (int di = -2; di <= 2; ++di) {
-        const double d = fabs(reg[kv][ry + dj][rx + di] - cv);
-        dev = (d > dev) ? d : dev;
+        dev = fmax(dev, fabs(reg[kv][ry + dj][rx + di] - cv));
       }
     const double tau = TAU_REL * (fabs(reg[FID_RHO][ry][rx]) + fabs(reg[FID_RHO_E][ry][rx]));
     inv_s = (float)(1.0 / ((dev >= tau) ? dev : tau));
     // shock sensor: lane group 0 on rho, group 1 on p; fires where
     // |q+ - 2 q0 + q-| > theta (q+ + 2 q0 + q-) along either axis
     if (grp < 2) {
+      const double(&q)[RY][RX] = (grp == 0) ? reg[FID_RHO] : pre;
+#pragma unroll
       for (int a = 0; a < 2; ++a) {
-        double q[3];
-        for (int t = 0; t < 3; ++t) {
-          const int yy = ry + (a ? t - 1 : 0), xx = rx + (a ? 0 : t - 1);
-          if (grp == 0) {
-            q[t] = reg[FID_RHO][yy][xx];
-          } else {
-            // p = (gamma - 1) (rho E - |rho u|^2 / (2 rho))
-            const double r0 = reg[FID_RHO][yy][xx], mx = reg[FID_RHO_U][yy][xx], my = reg[FID_RHO_V][yy][xx],
-                         mz = reg[FID_RHO_W][yy][xx];
-            q[t] = (GAMMA - 1.0) * (reg[FID_RHO_E][yy][xx] - 0.5 * ((mx * mx + my * my) + mz * mz) / r0);
-          }
-        }
-        const double num = fabs((q[2] - 2.0 * q[1]) + q[0]);
-        const double den = (q[2] + 2.0 * q[1]) + q[0];
+        const double qm = a ? q[ry - 1][rx] : q[ry][rx - 1];
+        const double q0 = q[ry][rx];
+        const double qp = a ? q[ry + 1][rx] : q[ry][rx + 1];
+        const double num = fabs((qp - 2.0 * q0) + qm);
+        const double den = (qp + 2.0 * q0) + qm;
         fires = fires || !(den > 0.0) || !(num <= run.theta * den);
       }
     }
@@ -721,7 +729,6 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   const bool need_plain_all = run.blend != 1.0;
   if (need_plain_all) plain_face(P);
 
-  double Fv[NF];
   bool pos = false;
   if (use_net && wave_live) {
     s4 xb[7];
@@ -780,6 +787,43 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
 #pragma unroll
     for (int k = 0; k < NF; ++k) Fv[k] = P[k];
   }
+  return learned;
+}
+
+// Whether the cell at unwrapped position (gx, gy) has its whole 5x5 stencil
+// inside the grid or on periodic axes.
+__device__ __forceinline__ bool stencil_inside(const GridDev &g, int gx, int gy) {
+  return (g.periodic[0] || (gx >= 2 && gx <= (int)g.n[0] - 3)) && (g.periodic[1] || (gy >= 2 && gy <= (int)g.n[1] - 3));
+}
+
+// The face-state kernel: one block = 16 x 4 cells, one wave per tile row;
+// writes the four face states of every cell (FS[(face * 5 + k) * fs + cell]).
+constexpr int RTX = 16, RTY = 4, RGX = RTX + 4, RGY = RTY + 4;
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, const double *U, const double *R,
+                                                       const double *Xin, Ctrl *ctrl, int mode) {
+  __shared__ double reg[NF][RGY][RGX];
+  __shared__ double pre[RGY][RGX];
+  __shared__ uint8_t rex[RGY][RGX];
+  __shared__ s4 sF[NFRAG * 64];
+  __shared__ float sB[2 * NH + NOUT];
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
+  const double dt = (STAGE == 2) ? step_dt(ctrl) : 0.0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
+  const int nx = (int)g.n[0], ny = (int)g.n[1];
+  const int64_t fs = g.fs;
+  const int x0 = blockIdx.x * RTX, y0 = blockIdx.y * RTY;
+  load_region<STAGE, RGX, RGY, 256>(g, x0 - 2, y0 - 2, U, R, Xin, dt, reg, rex, pre, tid);
+  load_net<256>(run, sF, sB, tid);
+  __syncthreads();
+
+  const int gx = x0 + c, gy = y0 + w;
+  const bool wave_live = gy < ny;
+  const bool live = wave_live && gx < nx;
+  const bool learnable = live && stencil_inside(g, gx, gy);
+  double Fv[NF];
+  const bool learned = cell_face<RGX, RGY>(g, run, reg, rex, pre, sF, sB, c + 2, w + 2, live, learnable, wave_live, lane, Fv);
   if (live) {
     const int64_t cell = gx + (int64_t)nx * gy;
 #pragma unroll
@@ -895,11 +939,201 @@ __global__ __launch_bounds__(256) void k_flux_fs(GridDev g, StageIn<STAGE> in, c
       out[k * fs + idx] = rk_update<STAGE>(g, u, x, rhs, in.dt);
     }
   }
-  block_max_to<256>(&ctrl->eig[slot], lam_max, threadIdx.x);
+  block_max_to<256>(ctrl, slot, lam_max, threadIdx.x);
+}
+
+// Rusanov flux of an axis-A face from the face states of its lower cell (that
+// cell's upper face, sl) and its upper cell (that cell's lower face, sr); a
+// side beyond a non-periodic border (el or er false) takes the border state
+// of the BC from the other side (euler.cpp:261-376).  Writes A F to af and
+// returns the face eigenvalue; counts non-positive states when count is set.
+template <int A>
+__device__ __forceinline__ double face_of_states(const GridDev &g, const double *sl, const double *sr, bool el,
+                                                 bool er, bool count, Ctrl *ctrl, double *af) {
+  double cl[NF], cr[NF];
+  if (!el) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) cr[k] = sr[k];
+    border_state(g, 2 * A, A, cr, cl);
+  } else if (!er) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) cl[k] = sl[k];
+    border_state(g, 2 * A + 1, A, cl, cr);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      cl[k] = sl[k];
+      cr[k] = sr[k];
+    }
+  }
+  Tup TL, TR;
+  make_tup(cl, TL);
+  make_tup(cr, TR);
+  double F[NF];
+  const double lam = face_flux(TL, TR, A, F);
+  if (count) count_neg(TL, TR, ctrl);
+  const double Ar = g.A;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) af[k] = Ar * F[k];
+  return lam;
+}
+
+// The fused stage kernel: one block = an 8 x 8 tile, six waves.  Waves 0-3
+// take the tile's cells (two rows each), wave 4 the ring columns left and
+// right of the tile, wave 5 the ring rows below and above it: the 32 cells
+// across the tile's border faces, whose face states the neighbouring blocks
+// also compute, bit for bit.  Face states stay in LDS; waves 0-1 then
+// evaluate the tile's x faces and waves 2-3 its y faces, each face once, and
+// every (cell, component) of the tile sums its residual in the reference
+// face order and writes its RK stage update: the same values as
+// k_recon_learned followed by k_flux_fs, in one launch and without the
+// face-state round trip through HBM.  The stage-3 launch of a step also
+// advances the time (its last block), in place of k_advance.
+constexpr int FT = 8, FR = FT + 6, FNT = 384;
+
+template <int STAGE>
+__global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, StageIn<STAGE> in, const double *U,
+                                                       double *out, Ctrl *ctrl, int mode) {
+  __shared__ double reg[NF][FR][FR];
+  __shared__ double pre[FR][FR];
+  __shared__ uint8_t rex[FR][FR];
+  __shared__ s4 sF[NFRAG * 64];
+  __shared__ float sB[2 * NH + NOUT];
+  __shared__ double fsl[FT + 2][FT + 2][4][NF];  // face states at region positions 2 .. FT + 3
+  __shared__ double afx[FT][FT + 1][NF];         // A F of x face i (left of tile column i) in tile row j
+  __shared__ double afy[FT][FT + 1][NF];         // A F of y face j (below tile row j) in tile column i
+  __shared__ int scount[2][4];
+  const int slot = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
+  if (STAGE >= 2) in.dt = step_dt(ctrl);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
+  const int nx = (int)g.n[0], ny = (int)g.n[1];
+  const int64_t fs = g.fs;
+  const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
+  const int mx = min(FT, nx - x0), my = min(FT, ny - y0);  // real columns and rows of the tile
+
+  // this thread's (cell, component) of the update; its inputs fetched early
+  const int uc = tid & 63, uk = tid >> 6;
+  const int ui = uc & 7, uj = uc >> 3;
+  const bool ulive = tid < 64 * NF && ui < mx && uj < my;
+  const int64_t uidx = (x0 + ui) + (int64_t)nx * (y0 + uj);
+  double ux = 0.0, uu = 0.0;
+  if (ulive && STAGE != 1) {
+    ux = (STAGE == 2) ? in.U[uk * fs + uidx] + div_vol(g, in.dt * in.R[uk * fs + uidx]) : in.X[uk * fs + uidx];
+    uu = U[uk * fs + uidx];
+  }
+
+  load_region<STAGE, FR, FR, FNT>(g, x0 - 3, y0 - 3, in.U, in.R, in.X, in.dt, reg, rex, pre, tid);
+  load_net<FNT>(run, sF, sB, tid);
+  __syncthreads();
+
+  // this lane's cell (region coordinates): tile rows 2w, 2w+1 (waves 0-3),
+  // ring columns at x0 - 1 and after the last tile column (wave 4), ring rows
+  // at y0 - 1 and after the last tile row (wave 5)
+  int rx, ry;
+  bool live;
+  if (w < 4) {
+    rx = 3 + (c & 7);
+    ry = 3 + 2 * w + (c >> 3);
+    live = (c & 7) < mx && 2 * w + (c >> 3) < my;
+  } else if (w == 4) {
+    rx = (c < 8) ? 2 : 3 + mx;
+    ry = 3 + (c & 7);
+    live = (c & 7) < my && rex[ry][rx];
+  } else {
+    rx = 3 + (c & 7);
+    ry = (c < 8) ? 2 : 3 + my;
+    live = (c & 7) < mx && rex[ry][rx];
+  }
+  const bool learnable = live && stencil_inside(g, x0 - 3 + rx, y0 - 3 + ry);
+  const bool wave_live = __ballot(live) != 0ull;
+  double Fv[NF];
+  const bool learned = cell_face<FR, FR>(g, run, reg, rex, pre, sF, sB, rx, ry, live, learnable, wave_live, lane, Fv);
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fsl[ry - 2][rx - 2][grp][k] = Fv[k];
+  }
+  if (STAGE == 1 && w < 4) {
+    const unsigned long long lm = __ballot(live && learned && grp == 0);
+    const unsigned long long fm = __ballot(live && !learned && grp == 0);
+    if (lane == 0) {
+      scount[0][w] = __popcll(lm);
+      scount[1][w] = __popcll(fm);
+    }
+  }
+  __syncthreads();
+
+  // each face of the tile once: a cell's upper face, and its lower face only
+  // at a border, counts the non-positive states (as the per-cell form does)
+  double lam_max = 0.0;
+  if (w < 2) {
+    const int j = tid / (FT + 1), i = tid % (FT + 1);
+    if (tid < FT * (FT + 1) && j < my && i <= mx) {
+      const bool el = rex[3 + j][2 + i], er = rex[3 + j][3 + i];
+      lam_max = face_of_states<0>(g, fsl[1 + j][i][1], fsl[1 + j][1 + i][0], el, er, i >= 1 || !el, ctrl, afx[j][i]);
+    }
+  } else if (w < 4) {
+    const int t = tid - 128;
+    const int i = t / (FT + 1), j = t % (FT + 1);
+    if (t < FT * (FT + 1) && i < mx && j <= my) {
+      const bool el = rex[2 + j][3 + i], er = rex[3 + j][3 + i];
+      lam_max = face_of_states<1>(g, fsl[j][1 + i][3], fsl[1 + j][1 + i][2], el, er, j >= 1 || !el, ctrl, afy[i][j]);
+    }
+  }
+  __syncthreads();
+
+  if (ulive) {
+    // reference face order; the wrap face first for the last cell of a periodic axis
+    const bool wrap_x = g.periodic[0] && x0 + ui == nx - 1;
+    const bool wrap_y = g.periodic[1] && y0 + uj == ny - 1;
+    const double xm = afx[uj][ui][uk], xp = afx[uj][ui + 1][uk];
+    const double ym = afy[ui][uj][uk], yp = afy[ui][uj + 1][uk];
+    double rhs = 0.0;
+    if (wrap_x) { rhs -= xp; rhs += xm; }
+    else { rhs += xm; rhs -= xp; }
+    if (wrap_y) { rhs -= yp; rhs += ym; }
+    else { rhs += ym; rhs -= yp; }
+    out[uk * fs + uidx] = rk_update<STAGE>(g, uu, ux, rhs, in.dt);
+  }
+  if (STAGE == 1 && tid == 0) {
+    const int64_t b = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    run.count[2 * b] += (unsigned long long)(scount[0][0] + scount[0][1] + scount[0][2] + scount[0][3]);
+    run.count[2 * b + 1] += (unsigned long long)(scount[1][0] + scount[1][1] + scount[1][2] + scount[1][3]);
+  }
+  block_max_to<FNT>(ctrl, slot, lam_max, tid);
+  // stage 3 of a step: the last block advances the time (no separate launch)
+  if (STAGE == 3 && mode == MODE_STEP) close_step(ctrl, tid);
 }
 
 int64_t learn_count_slots(const GridDev &g) {
-  return 2 * ((g.n[0] + RTX - 1) / RTX) * ((g.n[1] + RTY - 1) / RTY);
+  const int64_t a = ((g.n[0] + RTX - 1) / RTX) * ((g.n[1] + RTY - 1) / RTY);
+  const int64_t b = ((g.n[0] + FT - 1) / FT) * ((g.n[1] + FT - 1) / FT);
+  return 2 * (a > b ? a : b);
+}
+
+template <int STAGE>
+static hipError_t stage_learned_t(const GridDev &g, const LearnRun &run, int mode, const double *U, const double *R,
+                                  const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
+  StageIn<STAGE> in;
+  in.fs = g.fs;
+  in.dt = 0.0;
+  in.U = U;
+  in.R = R;
+  in.X = (STAGE == 3) ? W : U;
+  double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
+  dim3 grid((unsigned)((g.n[0] + FT - 1) / FT), (unsigned)((g.n[1] + FT - 1) / FT));
+  hipLaunchKernelGGL((k_stage_learned<STAGE>), grid, dim3(FNT), 0, s, g, run, in, U, out, ctrl, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_learned(const GridDev &g, const LearnRun &run, int stage, int mode, const double *U,
+                                const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
+                                hipStream_t s) {
+  switch (stage) {
+    case 1: return stage_learned_t<1>(g, run, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return stage_learned_t<2>(g, run, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return stage_learned_t<3>(g, run, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
 }
 
 template <int STAGE>

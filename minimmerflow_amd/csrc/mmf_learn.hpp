@@ -69,6 +69,10 @@ hipError_t launch_learn_collect(const GridDev &fine, const double *Uf, int facto
 hipError_t launch_learn_restrict(const GridDev &fine, const double *Uf, int factor, double *coarse, hipStream_t s);
 hipError_t launch_recon_learned(const GridDev &g, const LearnRun &run, int stage, int mode, const double *U,
                                 const double *R, const double *W, Ctrl *ctrl, hipStream_t s);
+// Fused learned stage (reconstruction + flux + RK update in one launch).
+hipError_t launch_stage_learned(const GridDev &g, const LearnRun &run, int stage, int mode, const double *U,
+                                const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
+                                hipStream_t s);
 hipError_t launch_flux_fs(const GridDev &g, const LearnRun &run, int stage, int mode, const double *U, const double *R,
                           const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s);
 

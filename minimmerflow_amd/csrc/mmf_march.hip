@@ -490,7 +490,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       }
     }
   }
-  block_max_to<NT>(&ctrl->eig[slot_eig], lam_max, tid);
+  block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
 }
 
 template <int DIM, int TY, int STAGE, bool MASK, bool P2>

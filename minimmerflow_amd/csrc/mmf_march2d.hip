@@ -372,7 +372,7 @@ __global__ __launch_bounds__(W2) void k_march2d(GridDev g, const double *X, cons
   In2<STAGE, P2> in{X, Uin, R, g.fs};
   double lam = march2d_item<STAGE, MASK, P2>(g, in, Uin, out, dt, ctrl, tile, m0, m1, lds, lfl);
   unsigned long long b = wave_max_bits(lam);
-  if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[(mode == MODE_RHS_ONLY) ? 3 : STAGE - 1], b);
+  if (threadIdx.x == 0) stage_max_put(ctrl, (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1, b);
 }
 
 template <int STAGE, bool MASK, bool P2>
@@ -681,7 +681,7 @@ __global__ __launch_bounds__(W2O) void k_march2d_o2(GridDev g, const double *X, 
     __syncthreads();
   }
   const unsigned long long b = wave_max_bits(lam_max);
-  if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[(mode == MODE_RHS_ONLY) ? 3 : STAGE - 1], b);
+  if (threadIdx.x == 0) stage_max_put(ctrl, (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1, b);
 }
 
 template <int STAGE, bool P2>
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(W2) void k_batch2d(const BatchCase *cases, const in
     else
       lam = march2d_item<STAGE, false, false>(g, In2<STAGE, false>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds, lfl);
     unsigned long long b = wave_max_bits(lam);
-    if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[STAGE - 1], b);
+    if (threadIdx.x == 0) stage_max_put(ctrl, STAGE - 1, b);
   }
 }
 
@@ -760,11 +760,12 @@ __global__ void k_batch_advance(const BatchCase *cases, int ncases, unsigned int
     c->t += dt;
     c->step += 1;
 #pragma unroll
-    for (int s = 0; s < 3; ++s) c->last_eig[s] = __longlong_as_double((long long)c->eig[s]);
+    for (int s = 0; s < 3; ++s) c->last_eig[s] = stage_max(c, s);
   }
-  c->eig[0] = 0ull;
-  c->eig[1] = 0ull;
-  c->eig[2] = 0ull;
+#pragma unroll
+  for (int s = 0; s < 3; ++s)
+#pragma unroll
+    for (int k = 0; k < EIG_SPLIT; ++k) c->eig[s][k][0] = 0ull;
 }
 
 hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,

@@ -13,12 +13,84 @@ namespace mmf {
 // division sequence.
 __device__ __forceinline__ double div_vol(const GridDev &g, double x) { return g.pow2V ? x * g.invV : x / g.V; }
 
+// Linear index of this block in its grid.
+__device__ __forceinline__ unsigned int block_linear() {
+  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+// Stage maximum s formed by earlier launches (plain loads; largest over the lines).
+__device__ __forceinline__ double stage_max(const Ctrl *c, int s) {
+  unsigned long long m = 0;
+#pragma unroll
+  for (int k = 0; k < EIG_SPLIT; ++k) m = (c->eig[s][k][0] > m) ? c->eig[s][k][0] : m;
+  return __longlong_as_double((long long)m);
+}
+
+// This block's contribution (bits of a non-negative double) to stage maximum s.
+__device__ __forceinline__ void stage_max_put(Ctrl *c, int s, unsigned long long bits) {
+  if (bits) atomicMax(&c->eig[s][block_linear() % EIG_SPLIT][0], bits);
+}
+
 // dt rule of main.cpp:398-402, evaluated identically by every thread.
 __device__ __forceinline__ double step_dt(const Ctrl *c) {
-  double me = __longlong_as_double((long long)c->eig[0]);
+  double me = stage_max(c, 0);
   double dt = 0.9 * c->cfl * c->hmin / me;
   if (c->t + dt > c->tmax) dt = c->tmax - c->t;
   return dt;
+}
+
+// The end of a step (main.cpp:505-506), by one whole wave: t += dt, the
+// stage maxima kept as the last step's and reset.  Lane 16 s + k reads line
+// k of stage s with an atomic, so a launch whose other blocks formed them may
+// call it.
+__device__ __forceinline__ void advance_ctrl_wave(Ctrl *c, int lane) {
+  unsigned long long v = 0;
+  if (lane < 3 * EIG_SPLIT) v = atomicAdd(&c->eig[lane / EIG_SPLIT][lane % EIG_SPLIT][0], 0ull);
+#pragma unroll
+  for (int off = EIG_SPLIT / 2; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = (o > v) ? o : v;
+  }
+  const unsigned long long e0 = __shfl(v, 0, 64), e1 = __shfl(v, EIG_SPLIT, 64), e2 = __shfl(v, 2 * EIG_SPLIT, 64);
+  if (lane == 0 && c->t < c->tmax) {
+    double dt = 0.9 * c->cfl * c->hmin / __longlong_as_double((long long)e0);
+    if (c->t + dt > c->tmax) dt = c->tmax - c->t;
+    c->dt = dt;
+    c->t += dt;
+    c->step += 1;
+    c->last_eig[0] = __longlong_as_double((long long)e0);
+    c->last_eig[1] = __longlong_as_double((long long)e1);
+    c->last_eig[2] = __longlong_as_double((long long)e2);
+  }
+  if (lane < 3 * EIG_SPLIT) atomicExch(&c->eig[lane / EIG_SPLIT][lane % EIG_SPLIT][0], 0ull);
+}
+
+// Called by every thread of every block at the very end of a launch that
+// closes a step, after block_max_to: the last block to finish advances the
+// time for the whole grid.  Thread 0 counts its block on its line only once
+// its stage-maximum atomic has completed (s_waitcnt 0; device-scope atomics
+// are performed at the point of coherence of all XCDs); the block completing
+// a line counts the line, and the block completing the last line reads every
+// block's maximum with its own atomics.  Only atomics carry the hand-over,
+// so no L2 write-back fence is needed.
+__device__ __forceinline__ void close_step(Ctrl *ctrl, int tid) {
+  __shared__ int last;
+  if (tid == 0) {
+    const unsigned int nb = gridDim.x * gridDim.y * gridDim.z, k = block_linear() % EIG_SPLIT;
+    const unsigned long long nk = (nb - k + EIG_SPLIT - 1) / EIG_SPLIT;  // blocks on line k
+    const unsigned long long nl = (nb < (unsigned)EIG_SPLIT) ? nb : EIG_SPLIT;  // lines in use
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    int l = 0;
+    if (atomicAdd(&ctrl->done[k][0], 1ull) == nk - 1) l = atomicAdd(&ctrl->done[EIG_SPLIT][0], 1ull) == nl - 1;
+    last = l;
+  }
+  __syncthreads();
+  if (last && tid < 64) {
+    advance_ctrl_wave(ctrl, tid);
+    if (tid <= EIG_SPLIT) atomicExch(&ctrl->done[tid][0], 0ull);
+  }
 }
 
 // Stage input state of one cell: U (stage 1), W1 = U + dt R / V rebuilt on the
@@ -41,9 +113,10 @@ struct StageIn {
   }
 };
 
-// Block max of non-negative doubles (compared as u64 bits), one atomic per block.
+// Block max of non-negative doubles (compared as u64 bits) into stage
+// maximum s, one atomic per block (on the block's line).
 template <int NT>
-__device__ __forceinline__ void block_max_to(unsigned long long *dst, double v, int tid) {
+__device__ __forceinline__ void block_max_to(Ctrl *ctrl, int s, double v, int tid) {
   __shared__ unsigned long long red[(NT + 63) / 64];
   unsigned long long b = dbits(v);
 #pragma unroll
@@ -58,7 +131,7 @@ __device__ __forceinline__ void block_max_to(unsigned long long *dst, double v, 
     unsigned long long m = red[0];
 #pragma unroll
     for (int w = 1; w < (NT + 63) / 64; ++w) m = (red[w] > m) ? red[w] : m;
-    if (m) atomicMax(dst, m);
+    stage_max_put(ctrl, s, m);
   }
 }
 

@@ -147,26 +147,73 @@ def test_face_state_kernels_equal_gather_order2(mmf, bc):
     U = random_state(mmf, n, 3)
     s.set_kernel(mmf.KERNEL_GATHER)
     ref = run(s, U)
-    s.set_learned(None, enable=True)
-    got = run(s, U)
+    # the split face-state + flux kernels (gather) and the fused stage kernel (auto)
+    for kind in (mmf.KERNEL_GATHER, mmf.KERNEL_AUTO):
+        s.set_kernel(kind)
+        s.set_learned(None, enable=True)
+        got = run(s, U)
+        for a, b in zip(got, ref):
+            if isinstance(a, np.ndarray):
+                assert np.array_equal(a, b), kind
+            else:
+                assert a == b, kind
+        net = mmf.Learner(seed=1)
+        s.set_learned(net, sensor_threshold=-1.0)  # every cell's sensor fires: all plain
+        s.reset_learned_counts()
+        got = run(s, U)
+        for a, b in zip(got, ref):
+            if isinstance(a, np.ndarray):
+                assert np.array_equal(a, b), kind
+            else:
+                assert a == b, kind
+        learned, fallback = s.learned_counts()
+        assert learned == 0 and fallback == n[0] * n[1] * (1 + 4), kind  # compute_rhs + 4 steps
+        s.set_learned(None, enable=False)
+        assert np.array_equal(run(s, U)[2], ref[2]), kind
+
+
+def smooth_state(mmf, n, seed):
+    """Smooth periodic fields (random phases): the sensor stays quiet and the
+    network sets most face values."""
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0, 2 * np.pi, 6)
+    x = (np.arange(n[0]) + 0.5) / n[0]
+    y = (np.arange(n[1]) + 0.5) / n[1]
+    X, Y = np.meshgrid(x, y)
+    tp = 2 * np.pi
+    prim = np.zeros((n[1], n[0], 5))
+    prim[..., 0] = 1.0 + 0.2 * np.sin(tp * X + ph[0]) * np.cos(tp * Y + ph[1])
+    prim[..., 1] = 0.3 * np.sin(tp * Y + ph[2])
+    prim[..., 2] = 0.3 * np.cos(tp * X + ph[3])
+    prim[..., 4] = 1.0 + 0.2 * np.cos(tp * X + ph[4]) * np.sin(tp * Y + ph[5])
+    return mmf.batch_prim2cons(prim.reshape(-1, 5))
+
+
+@pytest.mark.parametrize("n,bc", [([48, 48], [4, 4, 4, 4]), ([50, 37], [0, 1, 3, 2]), ([61, 20], [4, 4, 1, 1]),
+                                  ([7, 13], [4, 4, 0, 3]), ([130, 9], [1, 0, 4, 4]), ([8, 8], [4, 4, 4, 4])])
+def test_fused_stage_equals_split_kernels(mmf, n, bc):
+    """The fused stage kernel (8x8 tiles, ring cells recomputed, face states in
+    LDS) against the face-state + flux kernels with a network that sets most
+    faces: bit-identical RHS, states, time control, positivity counts and
+    network/fallback counts, including ragged tiles and periodic wraps."""
+    info = [[1.0, 0.2, -0.1, 0.0, 1.0]] * 6
+    s = mmf.Solver(2, n, [-1.0, -1.0, 0.0], 2.0 / max(n), bc + [0, 0], info, 2)
+    U = smooth_state(mmf, n, sum(n))
+    net, _ = scaled_net(mmf, 3, 0.05)
+    s.set_learned(net, sensor_threshold=0.05)
+    out = {}
+    for kind in (mmf.KERNEL_GATHER, mmf.KERNEL_AUTO):
+        s.set_kernel(kind)
+        s.reset_learned_counts()
+        out[kind] = run(s, U) + (s.learned_counts(),)
+    ref, got = out[mmf.KERNEL_GATHER], out[mmf.KERNEL_AUTO]
+    print("%s: network / fallback cell-stages %s" % (n, ref[-1]), flush=True)
+    assert ref[-1][0] > 0
     for a, b in zip(got, ref):
         if isinstance(a, np.ndarray):
             assert np.array_equal(a, b)
         else:
             assert a == b
-    net = mmf.Learner(seed=1)
-    s.set_learned(net, sensor_threshold=-1.0)  # every cell's sensor fires: all plain
-    s.reset_learned_counts()
-    got = run(s, U)
-    for a, b in zip(got, ref):
-        if isinstance(a, np.ndarray):
-            assert np.array_equal(a, b)
-        else:
-            assert a == b
-    learned, fallback = s.learned_counts()
-    assert learned == 0 and fallback == n[0] * n[1] * (1 + 4)  # compute_rhs + 4 steps
-    s.set_learned(None, enable=False)
-    assert np.array_equal(run(s, U)[2], ref[2])
 
 
 def scaled_net(mmf, seed, scale):
