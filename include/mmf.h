@@ -83,11 +83,16 @@ const char *mmf_status_string(int status);
 int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *ninterfaces);
 
 /* ---- stage-kernel selection ------------------------------------------------ */
-enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2 };
-/* AUTO: the marching LDS-tile kernels wherever they apply (order 1; order 2
- * in 2D without bodies), else the per-cell gather kernel.  tile_rows (3D: 4 or 8) and chunk (planes per
- * block along the last axis) shape the marching kernel; <= 0 keeps the
- * default.  Every variant gives bit-identical results. */
+enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2, MMF_KERNEL_TILE = 3 };
+/* GATHER: per-cell kernel, any grid.  MARCH: the marching LDS-tile kernels
+ * (order 1; order 2 in 2D without bodies).  TILE: the 8x8 LDS tile kernel of
+ * 2D order-2 grids without bodies (each face once, the step's time advance
+ * in its last stage; also the learned reconstruction's kernel).  AUTO: TILE
+ * for 2D order-2 grids without bodies below 2^18 cells, MARCH wherever else
+ * it applies (2D order 2 from 2^20 cells), else GATHER.  mmf_set_kernel refuses MARCH or TILE on a grid they do not
+ * apply to; bodies set afterwards make them fall back to GATHER.  tile_rows (3D: 4 or 8) and chunk (planes per block along the last axis)
+ * shape the marching kernel; <= 0 keeps the default.  Every variant gives
+ * bit-identical results. */
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk);
 /* kind actually used for the next launches, and the marching shape */
 int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk);

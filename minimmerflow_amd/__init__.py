@@ -24,7 +24,7 @@ NF = 5
 OK, ERR_INVALID, ERR_UNSUPPORTED_ORDER, ERR_HIP, ERR_NO_DEVICE, ERR_OOM, ERR_STATE = range(7)
 BC_NONE, BC_FREE_FLOW, BC_REFLECTING, BC_WALL, BC_DIRICHLET, BC_PERIODIC = -1, 0, 1, 2, 3, 4
 CELL_FLUID, CELL_SOLVED = 1, 2
-KERNEL_AUTO, KERNEL_GATHER, KERNEL_MARCH = 0, 1, 2
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_MARCH, KERNEL_TILE = 0, 1, 2, 3
 (VORTEX_XY, VORTEX_ZX, VORTEX_YZ, RADSOD, SOD_X, SOD_Y, SOD_Z, FFSTEP, FREESTREAM) = range(9)
 PROBLEM_KEYS = {"vortex_xy": VORTEX_XY, "vortex_zx": VORTEX_ZX, "vortex_yz": VORTEX_YZ,
                 "radsod": RADSOD, "sod3d_x": SOD_X, "sod3d_y": SOD_Y, "sod3d_z": SOD_Z,

@@ -78,6 +78,23 @@ bool use_march(const mmf_ctx *ctx) {
   return true;
 }
 
+bool tile_applies(const mmf_ctx *ctx) { return ctx->g.dim == 2 && ctx->g.order == 2 && !ctx->g.flags; }
+
+// AUTO takes the tile kernel for 2D order-2 grids below this size (measured on
+// MI355X, vortex: 128^2 25.7 vs 50.0 us per step for the gather kernel, 256^2
+// 48.5 vs 58.0; at 512^2 the gather and strip kernels are faster, 125-128 vs
+// 146, the tile kernel's fixed per-block latency no longer hidden).
+constexpr int64_t TILE_AUTO_MAX_N = (int64_t)1 << 18;
+
+// The stage kernel of the next launches.  With the learned reconstruction on:
+// the tile kernel, or (GATHER) the face-state + flux kernel pair.
+int kernel_used(const mmf_ctx *ctx) {
+  if (ctx->learn_on) return ctx->kernel == mmf::KERNEL_GATHER ? mmf::KERNEL_GATHER : mmf::KERNEL_TILE;
+  if (ctx->kernel == mmf::KERNEL_TILE) return tile_applies(ctx) ? mmf::KERNEL_TILE : mmf::KERNEL_GATHER;
+  if (ctx->kernel == mmf::KERNEL_AUTO && tile_applies(ctx) && ctx->g.N < TILE_AUTO_MAX_N) return mmf::KERNEL_TILE;
+  return use_march(ctx) ? mmf::KERNEL_MARCH : mmf::KERNEL_GATHER;
+}
+
 int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->timing) {
@@ -86,22 +103,25 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
   }
   hipError_t e;
-  if (ctx->learn_on && ctx->kernel == mmf::KERNEL_GATHER) {
+  const int kind = kernel_used(ctx);
+  if (ctx->learn_on && kind == mmf::KERNEL_GATHER) {
     // face states (learned or plain) through HBM, then flux + RK stage update
     e = mmf::launch_recon_learned(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dctrl, ctx->stream);
     if (e == hipSuccess)
       e = mmf::launch_flux_fs(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
                               ctx->dctrl, ctx->stream);
-  } else if (ctx->learn_on) {
-    // the same in one launch per stage (face states in LDS)
-    e = mmf::launch_stage_learned(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW,
-                                  ctx->dU, ctx->dctrl, ctx->stream);
+  } else if (kind == mmf::KERNEL_TILE) {
+    // one launch per stage, face states in LDS; plain face values without a network
+    mmf::LearnRun plain{};
+    plain.blend = 1.0;
+    e = mmf::launch_stage_learned(ctx->g, ctx->learn_on ? ctx->learn : plain, stage, mode, ctx->dU, ctx->dR, ctx->dW,
+                                  ctx->dR, ctx->dW, ctx->dU, ctx->dctrl, ctx->stream);
+  } else if (kind == mmf::KERNEL_MARCH) {
+    e = mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
+                                ctx->dctrl, ctx->stream);
   } else {
-    const bool march = use_march(ctx);
-    e = march ? mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW,
-                                        ctx->dU, ctx->dctrl, ctx->stream)
-              : mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
-                                  ctx->dctrl, ctx->stream);
+    e = mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU, ctx->dctrl,
+                          ctx->stream);
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "stage kernel launch");
   if (ctx->timing) {
@@ -247,7 +267,9 @@ int mmf_destroy(mmf_ctx *ctx) {
 
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk) {
   if (!ctx) return MMF_ERR_INVALID;
-  if (kind < mmf::KERNEL_AUTO || kind > mmf::KERNEL_MARCH) return fail(ctx, MMF_ERR_INVALID, "unknown kernel kind");
+  if (kind < mmf::KERNEL_AUTO || kind > mmf::KERNEL_TILE) return fail(ctx, MMF_ERR_INVALID, "unknown kernel kind");
+  if (kind == mmf::KERNEL_TILE && !tile_applies(ctx))
+    return fail(ctx, MMF_ERR_INVALID, "tile kernel: order 2 in 2D without bodies");
   mmf::MarchShape sh = mmf::default_march_shape(ctx->g, ctx->num_cus);
   if (tile_rows > 0) sh.ty = tile_rows;
   if (chunk > 0) sh.chunk = chunk;
@@ -260,8 +282,7 @@ int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk)
 
 int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk) {
   if (!ctx) return MMF_ERR_INVALID;
-  const bool march = use_march(ctx);
-  if (kind) *kind = march ? mmf::KERNEL_MARCH : mmf::KERNEL_GATHER;
+  if (kind) *kind = kernel_used(ctx);
   if (tile_rows) *tile_rows = ctx->shape.ty;
   if (chunk) *chunk = ctx->shape.chunk;
   return MMF_OK;
@@ -455,8 +476,8 @@ int enqueue_steps(mmf_ctx *ctx, int64_t nsteps) {
       int rc = launch_stage_timed(ctx, stage, mmf::MODE_STEP);
       if (rc != MMF_OK) return rc;
     }
-    // the fused learned stage kernel advances the time itself
-    if (ctx->learn_on && ctx->kernel != mmf::KERNEL_GATHER) continue;
+    // the tile kernel's stage-3 launch advances the time itself
+    if (kernel_used(ctx) == mmf::KERNEL_TILE) continue;
     hipError_t e = mmf::launch_advance(ctx->dctrl, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "advance kernel");
   }

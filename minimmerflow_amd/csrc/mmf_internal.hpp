@@ -28,7 +28,7 @@ struct GridDev {
 };
 
 // Stage-kernel variants.
-enum KernelKind { KERNEL_AUTO = 0, KERNEL_GATHER = 1, KERNEL_MARCH = 2 };
+enum KernelKind { KERNEL_AUTO = 0, KERNEL_GATHER = 1, KERNEL_MARCH = 2, KERNEL_TILE = 3 };
 
 // Launch shape of the marching kernel (tile rows and march chunk).
 struct MarchShape {

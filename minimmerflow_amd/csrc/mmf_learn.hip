@@ -621,10 +621,12 @@ hipError_t launch_learn_restrict(const GridDev &fine, const double *Uf, int fact
 // Stage state of an RX x RY region whose first cell is (xo, yo), wrapped on
 // periodic axes; ex = 0 (and zero state) where a cell lies beyond a
 // non-periodic border.  Stage 2 rebuilds W1 = U + dt R / V (main.cpp:421).
+// With want_p, pre ([RY][RX]) receives the pressure of every cell (the shock
+// sensor's).
 template <int STAGE, int RX, int RY, int NT>
 __device__ __forceinline__ void load_region(const GridDev &g, int xo, int yo, const double *U, const double *R,
                                             const double *Xin, double dt, double (&reg)[NF][RY][RX],
-                                            uint8_t (&rex)[RY][RX], double (&pre)[RY][RX], int tid) {
+                                            uint8_t (&rex)[RY][RX], double *pre, bool want_p, int tid) {
   const int nx = (int)g.n[0], ny = (int)g.n[1];
   const int64_t fs = g.fs;
   for (int e = tid; e < RX * RY; e += NT) {
@@ -642,8 +644,10 @@ __device__ __forceinline__ void load_region(const GridDev &g, int xo, int yo, co
       reg[k][ry][rx] = v[k];
     }
     // p = (gamma - 1) (rho E - |rho u|^2 / (2 rho)) for the shock sensor
-    const double m2 = (v[FID_RHO_U] * v[FID_RHO_U] + v[FID_RHO_V] * v[FID_RHO_V]) + v[FID_RHO_W] * v[FID_RHO_W];
-    pre[ry][rx] = ex ? (GAMMA - 1.0) * (v[FID_RHO_E] - 0.5 * m2 / v[FID_RHO]) : 0.0;
+    if (want_p) {
+      const double m2 = (v[FID_RHO_U] * v[FID_RHO_U] + v[FID_RHO_V] * v[FID_RHO_V]) + v[FID_RHO_W] * v[FID_RHO_W];
+      pre[ry * RX + rx] = ex ? (GAMMA - 1.0) * (v[FID_RHO_E] - 0.5 * m2 / v[FID_RHO]) : 0.0;
+    }
   }
 }
 
@@ -666,13 +670,13 @@ __device__ __forceinline__ void load_net(const LearnRun &run, s4 *sF, float *sB,
 // live (wave_live, uniform over the wave).  Returns whether the cell's faces
 // are the learned ones.  Every cell's result depends only on its own
 // stencil, so any kernel that places the cell in some lane gets the same bits.
-template <int RX, int RY>
+template <bool NET, int RX, int RY>
 __device__ __forceinline__ bool cell_face(const GridDev &g, const LearnRun &run, const double (&reg)[NF][RY][RX],
-                                          const uint8_t (&rex)[RY][RX], const double (&pre)[RY][RX], const s4 *sF,
+                                          const uint8_t (&rex)[RY][RX], const double *pre, const s4 *sF,
                                           const float *sB, int rx, int ry, bool live, bool learnable, bool wave_live,
                                           int lane, double *Fv) {
   const int c = lane & 15, grp = lane >> 4;
-  const bool use_net = run.net.frags != nullptr;
+  const bool use_net = NET && run.net.frags != nullptr;
   bool fires = false;
   // this lane's stencil variable (lane group grp), its centre and 1 / scale
   const int kv = var_of(grp);
@@ -692,12 +696,12 @@ __device__ __forceinline__ bool cell_face(const GridDev &g, const LearnRun &run,
     // shock sensor: lane group 0 on rho, group 1 on p; fires where
     // |q+ - 2 q0 + q-| > theta (q+ + 2 q0 + q-) along either axis
     if (grp < 2) {
-      const double(&q)[RY][RX] = (grp == 0) ? reg[FID_RHO] : pre;
+      const double *q = (grp == 0) ? &reg[FID_RHO][0][0] : pre;
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const double qm = a ? q[ry - 1][rx] : q[ry][rx - 1];
-        const double q0 = q[ry][rx];
-        const double qp = a ? q[ry + 1][rx] : q[ry][rx + 1];
+        const double qm = a ? q[(ry - 1) * RX + rx] : q[ry * RX + rx - 1];
+        const double q0 = q[ry * RX + rx];
+        const double qp = a ? q[(ry + 1) * RX + rx] : q[ry * RX + rx + 1];
         const double num = fabs((qp - 2.0 * q0) + qm);
         const double den = (qp + 2.0 * q0) + qm;
         fires = fires || !(den > 0.0) || !(num <= run.theta * den);
@@ -814,7 +818,8 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   const int nx = (int)g.n[0], ny = (int)g.n[1];
   const int64_t fs = g.fs;
   const int x0 = blockIdx.x * RTX, y0 = blockIdx.y * RTY;
-  load_region<STAGE, RGX, RGY, 256>(g, x0 - 2, y0 - 2, U, R, Xin, dt, reg, rex, pre, tid);
+  load_region<STAGE, RGX, RGY, 256>(g, x0 - 2, y0 - 2, U, R, Xin, dt, reg, rex, &pre[0][0], run.net.frags != nullptr,
+                                    tid);
   load_net<256>(run, sF, sB, tid);
   __syncthreads();
 
@@ -823,7 +828,8 @@ __global__ __launch_bounds__(256) void k_recon_learned(GridDev g, LearnRun run, 
   const bool live = wave_live && gx < nx;
   const bool learnable = live && stencil_inside(g, gx, gy);
   double Fv[NF];
-  const bool learned = cell_face<RGX, RGY>(g, run, reg, rex, pre, sF, sB, c + 2, w + 2, live, learnable, wave_live, lane, Fv);
+  const bool learned =
+      cell_face<true, RGX, RGY>(g, run, reg, rex, &pre[0][0], sF, sB, c + 2, w + 2, live, learnable, wave_live, lane, Fv);
   if (live) {
     const int64_t cell = gx + (int64_t)nx * gy;
 #pragma unroll
@@ -978,7 +984,8 @@ __device__ __forceinline__ double face_of_states(const GridDev &g, const double 
   return lam;
 }
 
-// The fused stage kernel: one block = an 8 x 8 tile, six waves.  Waves 0-3
+// The tile stage kernel of 2D order-2 grids (plain face values without a
+// network, learned ones with): one block = an 8 x 8 tile, six waves.  Waves 0-3
 // take the tile's cells (two rows each), wave 4 the ring columns left and
 // right of the tile, wave 5 the ring rows below and above it: the 32 cells
 // across the tile's border faces, whose face states the neighbouring blocks
@@ -991,14 +998,15 @@ __device__ __forceinline__ double face_of_states(const GridDev &g, const double 
 // advances the time (its last block), in place of k_advance.
 constexpr int FT = 8, FR = FT + 6, FNT = 384;
 
-template <int STAGE>
+template <int STAGE, bool NET>
 __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, StageIn<STAGE> in, const double *U,
                                                        double *out, Ctrl *ctrl, int mode) {
+  // without a network (NET false) the weights and the sensor's pressure take no LDS
   __shared__ double reg[NF][FR][FR];
-  __shared__ double pre[FR][FR];
+  __shared__ double pre[NET ? FR * FR : 1];
   __shared__ uint8_t rex[FR][FR];
-  __shared__ s4 sF[NFRAG * 64];
-  __shared__ float sB[2 * NH + NOUT];
+  __shared__ s4 sF[NET ? NFRAG * 64 : 1];
+  __shared__ float sB[NET ? 2 * NH + NOUT : 1];
   __shared__ double fsl[FT + 2][FT + 2][4][NF];  // face states at region positions 2 .. FT + 3
   __shared__ double afx[FT][FT + 1][NF];         // A F of x face i (left of tile column i) in tile row j
   __shared__ double afy[FT][FT + 1][NF];         // A F of y face j (below tile row j) in tile column i
@@ -1023,8 +1031,9 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
     uu = U[uk * fs + uidx];
   }
 
-  load_region<STAGE, FR, FR, FNT>(g, x0 - 3, y0 - 3, in.U, in.R, in.X, in.dt, reg, rex, pre, tid);
-  load_net<FNT>(run, sF, sB, tid);
+  load_region<STAGE, FR, FR, FNT>(g, x0 - 3, y0 - 3, in.U, in.R, in.X, in.dt, reg, rex, pre,
+                                  NET && run.net.frags != nullptr, tid);
+  if (NET) load_net<FNT>(run, sF, sB, tid);
   __syncthreads();
 
   // this lane's cell (region coordinates): tile rows 2w, 2w+1 (waves 0-3),
@@ -1048,7 +1057,7 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
   const bool learnable = live && stencil_inside(g, x0 - 3 + rx, y0 - 3 + ry);
   const bool wave_live = __ballot(live) != 0ull;
   double Fv[NF];
-  const bool learned = cell_face<FR, FR>(g, run, reg, rex, pre, sF, sB, rx, ry, live, learnable, wave_live, lane, Fv);
+  const bool learned = cell_face<NET, FR, FR>(g, run, reg, rex, pre, sF, sB, rx, ry, live, learnable, wave_live, lane, Fv);
   if (live) {
 #pragma unroll
     for (int k = 0; k < NF; ++k) fsl[ry - 2][rx - 2][grp][k] = Fv[k];
@@ -1095,7 +1104,7 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
     else { rhs += ym; rhs -= yp; }
     out[uk * fs + uidx] = rk_update<STAGE>(g, uu, ux, rhs, in.dt);
   }
-  if (STAGE == 1 && tid == 0) {
+  if (STAGE == 1 && tid == 0 && run.count != nullptr) {
     const int64_t b = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
     run.count[2 * b] += (unsigned long long)(scount[0][0] + scount[0][1] + scount[0][2] + scount[0][3]);
     run.count[2 * b + 1] += (unsigned long long)(scount[1][0] + scount[1][1] + scount[1][2] + scount[1][3]);
@@ -1122,7 +1131,10 @@ static hipError_t stage_learned_t(const GridDev &g, const LearnRun &run, int mod
   in.X = (STAGE == 3) ? W : U;
   double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
   dim3 grid((unsigned)((g.n[0] + FT - 1) / FT), (unsigned)((g.n[1] + FT - 1) / FT));
-  hipLaunchKernelGGL((k_stage_learned<STAGE>), grid, dim3(FNT), 0, s, g, run, in, U, out, ctrl, mode);
+  if (run.net.frags != nullptr)
+    hipLaunchKernelGGL((k_stage_learned<STAGE, true>), grid, dim3(FNT), 0, s, g, run, in, U, out, ctrl, mode);
+  else
+    hipLaunchKernelGGL((k_stage_learned<STAGE, false>), grid, dim3(FNT), 0, s, g, run, in, U, out, ctrl, mode);
   return hipGetLastError();
 }
 

@@ -127,3 +127,33 @@ def test_march_order2_2d_equals_gather(mmf, orc, name, n, bc):
     Ro, mo = o.compute_rhs(U)
     assert np.max(np.abs(ref[0] - Ro)) <= 1e-12 * np.max(np.abs(Ro))
     assert ref[1] == mo
+
+
+@pytest.mark.parametrize("name,n,bc", O2_CASES, ids=[c[0] for c in O2_CASES])
+def test_tile_order2_2d_equals_gather(mmf, name, n, bc):
+    """The 8x8 LDS tile kernel (ring cells recomputed, each face once, the
+    time advance in the stage-3 launch) against the per-cell gather kernel:
+    bit-identical RHS, steps, time control and positivity counts, also when
+    chosen by AUTO."""
+    info = np.zeros((6, 5))
+    info[:] = [1.0, 0.3, -0.1, 0.0, 1.0]
+    s, U = build(mmf, 2, list(n), bc, info, None, order=2, seed=5)
+    ref = run_variant(mmf, s, U, mmf.KERNEL_GATHER)
+    assert s.kernel()[0] == mmf.KERNEL_GATHER
+    for kind in (mmf.KERNEL_TILE, mmf.KERNEL_AUTO):
+        got = run_variant(mmf, s, U, kind)
+        assert s.kernel()[0] == mmf.KERNEL_TILE
+        assert np.array_equal(got[0], ref[0]), (name, kind, "rhs")
+        assert got[1] == ref[1], (name, kind, "maxEig")
+        assert np.array_equal(got[2], ref[2]), (name, kind, "state after 3 steps")
+        assert got[3] == ref[3], (name, kind, "time control")
+        assert got[4] == ref[4], (name, kind, "positivity counters")
+
+
+def test_tile_kernel_refused_where_it_does_not_apply(mmf):
+    s = mmf.Solver(2, [16, 16], [-1.0, -1.0, 0.0], 0.125, [0] * 6, None, 1)
+    with pytest.raises(mmf.MMFError):
+        s.set_kernel(mmf.KERNEL_TILE)
+    s3 = mmf.Solver(3, [8, 8, 8], [-1.0, -1.0, -1.0], 0.25, [0] * 6, None, 2)
+    with pytest.raises(mmf.MMFError):
+        s3.set_kernel(mmf.KERNEL_TILE)
