@@ -103,35 +103,34 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 // 16 cells per wave through the network on the matrix cores.  Lane l holds
 // cell l%16 and input group g = l/16: xb[kb] = inputs 16 kb + 4 g + (0..3).
-// Returns y[i] = output 4 g + i (face g, variable i) of cell l%16.
-// B: biases b1 (64), b2 (64), b3 (16).
+// Returns y[i] = output 4 g + i (face g, variable i) of cell l%16.  The
+// biases (B: b1 (64), b2 (64), b3 (16)) initialise the accumulators.
 __device__ __forceinline__ f4 mlp16(const s4 *F, const float *B, const s4 *xb, int lane) {
   const int g = lane >> 4;
   s4 h1[4], h2[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const float *b = B + 16 * mt + 4 * g;
+    f4 acc = {b[0], b[1], b[2], b[3]};  // the biases start the sums
 #pragma unroll
     for (int kb = 0; kb < 7; ++kb)
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(mt * 7 + kb) * 64 + lane], xb[kb], acc, 0, 0, 0);
-    const float *b = B + 16 * mt + 4 * g;
-    h1[mt] = bf4(gelu_f(acc[0] + b[0]), gelu_f(acc[1] + b[1]), gelu_f(acc[2] + b[2]), gelu_f(acc[3] + b[3]));
+    h1[mt] = bf4(gelu_f(acc[0]), gelu_f(acc[1]), gelu_f(acc[2]), gelu_f(acc[3]));
   }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const float *b = B + NH + 16 * mt + 4 * g;
+    f4 acc = {b[0], b[1], b[2], b[3]};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb)
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(28 + mt * 4 + kb) * 64 + lane], h1[kb], acc, 0, 0, 0);
-    const float *b = B + NH + 16 * mt + 4 * g;
-    h2[mt] = bf4(gelu_f(acc[0] + b[0]), gelu_f(acc[1] + b[1]), gelu_f(acc[2] + b[2]), gelu_f(acc[3] + b[3]));
+    h2[mt] = bf4(gelu_f(acc[0]), gelu_f(acc[1]), gelu_f(acc[2]), gelu_f(acc[3]));
   }
-  f4 y = {0.f, 0.f, 0.f, 0.f};
+  const float *b = B + 2 * NH + 4 * g;
+  f4 y = {b[0], b[1], b[2], b[3]};
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb)
     y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F[(44 + kb) * 64 + lane], h2[kb], y, 0, 0, 0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) y[i] += B[2 * NH + 4 * g + i];
   return y;
 }
 
