@@ -281,7 +281,7 @@ def run_learned(args, mmf):
             % (c["kind"], n, P1["l1"], 2 * n, out["plain%d" % (2 * n)]["l1"], 4 * n, out["plain%d" % (4 * n)]["l1"],
                n, L["l1"], L["wall_s"], 2 * n, out["plain%d" % (2 * n)]["wall_s"]))
         cases.append(out)
-    # stage timing of the learned reconstruction + flux kernels (HIP events)
+    # stage timing of the learned tile kernel (HIP events)
     c = LR.case_params(1, seed=99)[0]
     sv, _ = LR.solver(c, n)
     sv.set_learned(net, 0.05)
@@ -317,7 +317,8 @@ def run_learned(args, mmf):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": achieved / 2500.0, "traffic": None,
-                     "kernel": "k_recon_learned + k_flux_fs (one RK stage), network FLOPs only",
+                     "kernel": "k_stage_learned (tile kernel with the network: reconstruction, flux and RK "
+                               "update of one stage), network FLOPs only",
                      "algorithmic_flops_per_launch": flops, "avg_launch_ms": avg},
         "cpu_baseline": cpu,
     }
@@ -342,7 +343,7 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="radsod256")
     ap.add_argument("--n", type=int, default=0, help="cells per axis (default: the workload's)")
     ap.add_argument("--order", type=int, default=1, choices=(1, 2))
-    ap.add_argument("--kernel", choices=("auto", "gather", "march"), default="auto")
+    ap.add_argument("--kernel", choices=("auto", "gather", "march", "tile"), default="auto")
     ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4 or 8)")
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
@@ -380,7 +381,8 @@ def main():
     N = n ** 3
     t_setup = time.perf_counter()
     s, pb, mask_bytes, cfg = setup(mmf, n, args.order)
-    kind = {"auto": mmf.KERNEL_AUTO, "gather": mmf.KERNEL_GATHER, "march": mmf.KERNEL_MARCH}[args.kernel]
+    kind = {"auto": mmf.KERNEL_AUTO, "gather": mmf.KERNEL_GATHER, "march": mmf.KERNEL_MARCH,
+            "tile": mmf.KERNEL_TILE}[args.kernel]
     s.set_kernel(kind, args.rows, args.chunk)
     kinfo = s.kernel()
     solved = N if mask_bytes == 0 else int(((s.cell_flags() & mmf.CELL_SOLVED) != 0).sum())
