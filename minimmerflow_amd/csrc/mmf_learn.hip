@@ -995,34 +995,46 @@ __device__ __forceinline__ double face_of_states(const GridDev &g, const double 
 // k_recon_learned followed by k_flux_fs, in one launch and without the
 // face-state round trip through HBM.  The stage-3 launch of a step also
 // advances the time (its last block), in place of k_advance.
-constexpr int FT = 8, FR = FT + 6, FNT = 384;
+constexpr int TW = 8;  // tile width (two tile rows per wave of 16 cells)
+constexpr int TH = 8;  // tile height of the launched kernel
 
-template <int STAGE, bool NET>
-__global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, StageIn<STAGE> in, const double *U,
-                                                       double *out, Ctrl *ctrl, int mode) {
+template <int H>
+struct TileShape {
+  static constexpr int RX = TW + 6, RY = H + 6;  // region with a three-cell halo
+  static constexpr int NTW = H / 2;              // waves on the tile's cells
+  static constexpr int NT = 64 * (NTW + 2);       // + one wave each for ring columns and rows
+};
+
+template <int STAGE, bool NET, int H>
+__global__ __launch_bounds__(TileShape<H>::NT) void k_stage_learned(GridDev g, LearnRun run, StageIn<STAGE> in,
+                                                                     const double *U, double *out, Ctrl *ctrl,
+                                                                     int mode) {
+  using S = TileShape<H>;
+  constexpr int RX = S::RX, RY = S::RY, NTW = S::NTW, NT = S::NT;
   // without a network (NET false) the weights and the sensor's pressure take no LDS
-  __shared__ double reg[NF][FR][FR];
-  __shared__ double pre[NET ? FR * FR : 1];
-  __shared__ uint8_t rex[FR][FR];
+  __shared__ double reg[NF][RY][RX];
+  __shared__ double pre[NET ? RY * RX : 1];
+  __shared__ uint8_t rex[RY][RX];
   __shared__ s4 sF[NET ? NFRAG * 64 : 1];
   __shared__ float sB[NET ? 2 * NH + NOUT : 1];
-  __shared__ double fsl[FT + 2][FT + 2][4][NF];  // face states at region positions 2 .. FT + 3
-  __shared__ double afx[FT][FT + 1][NF];         // A F of x face i (left of tile column i) in tile row j
-  __shared__ double afy[FT][FT + 1][NF];         // A F of y face j (below tile row j) in tile column i
-  __shared__ int scount[2][4];
+  __shared__ double fsl[H + 2][TW + 2][4][NF];  // face states at region positions 2 .. T + 3
+  __shared__ double afx[H][TW + 1][NF];         // A F of x face i (left of tile column i) in tile row j
+  __shared__ double afy[TW][H + 1][NF];         // A F of y face j (below tile row j) in tile column i
+  __shared__ int scount[2][NTW];
   const int slot = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // uniform over the grid
   if (STAGE >= 2) in.dt = step_dt(ctrl);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, grp = lane >> 4;
   const int nx = (int)g.n[0], ny = (int)g.n[1];
   const int64_t fs = g.fs;
-  const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
-  const int mx = min(FT, nx - x0), my = min(FT, ny - y0);  // real columns and rows of the tile
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * H;
+  const int mx = min(TW, nx - x0), my = min(H, ny - y0);  // real columns and rows of the tile
 
   // this thread's (cell, component) of the update; its inputs fetched early
-  const int uc = tid & 63, uk = tid >> 6;
+  constexpr int NC = TW * H;
+  const int uc = tid % NC, uk = tid / NC;
   const int ui = uc & 7, uj = uc >> 3;
-  const bool ulive = tid < 64 * NF && ui < mx && uj < my;
+  const bool ulive = tid < NC * NF && ui < mx && uj < my;
   const int64_t uidx = (x0 + ui) + (int64_t)nx * (y0 + uj);
   double ux = 0.0, uu = 0.0;
   if (ulive && STAGE != 1) {
@@ -1030,24 +1042,26 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
     uu = U[uk * fs + uidx];
   }
 
-  load_region<STAGE, FR, FR, FNT>(g, x0 - 3, y0 - 3, in.U, in.R, in.X, in.dt, reg, rex, pre,
-                                  NET && run.net.frags != nullptr, tid);
-  if (NET) load_net<FNT>(run, sF, sB, tid);
+  load_region<STAGE, RX, RY, NT>(g, x0 - 3, y0 - 3, in.U, in.R, in.X, in.dt, reg, rex, pre,
+                                 NET && run.net.frags != nullptr, tid);
+  if (NET) load_net<NT>(run, sF, sB, tid);
   __syncthreads();
 
-  // this lane's cell (region coordinates): tile rows 2w, 2w+1 (waves 0-3),
-  // ring columns at x0 - 1 and after the last tile column (wave 4), ring rows
-  // at y0 - 1 and after the last tile row (wave 5)
+  // this lane's cell (region coordinates): tile rows 2w, 2w+1 (waves below
+  // NTW), ring columns at x0 - 1 and after the last tile column (wave NTW),
+  // ring rows at y0 - 1 and after the last tile row (wave NTW + 1); lanes
+  // without a cell keep coordinates inside the region
   int rx, ry;
   bool live;
-  if (w < 4) {
+  if (w < NTW) {
     rx = 3 + (c & 7);
     ry = 3 + 2 * w + (c >> 3);
     live = (c & 7) < mx && 2 * w + (c >> 3) < my;
-  } else if (w == 4) {
+  } else if (w == NTW) {
+    const int j = c & 7;
     rx = (c < 8) ? 2 : 3 + mx;
-    ry = 3 + (c & 7);
-    live = (c & 7) < my && rex[ry][rx];
+    ry = 3 + ((j < H) ? j : H - 1);
+    live = j < my && rex[ry][rx];
   } else {
     rx = 3 + (c & 7);
     ry = (c < 8) ? 2 : 3 + my;
@@ -1056,12 +1070,12 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
   const bool learnable = live && stencil_inside(g, x0 - 3 + rx, y0 - 3 + ry);
   const bool wave_live = __ballot(live) != 0ull;
   double Fv[NF];
-  const bool learned = cell_face<NET, FR, FR>(g, run, reg, rex, pre, sF, sB, rx, ry, live, learnable, wave_live, lane, Fv);
+  const bool learned = cell_face<NET, RX, RY>(g, run, reg, rex, pre, sF, sB, rx, ry, live, learnable, wave_live, lane, Fv);
   if (live) {
 #pragma unroll
     for (int k = 0; k < NF; ++k) fsl[ry - 2][rx - 2][grp][k] = Fv[k];
   }
-  if (STAGE == 1 && w < 4) {
+  if (STAGE == 1 && w < NTW) {
     const unsigned long long lm = __ballot(live && learned && grp == 0);
     const unsigned long long fm = __ballot(live && !learned && grp == 0);
     if (lane == 0) {
@@ -1071,19 +1085,22 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
   }
   __syncthreads();
 
-  // each face of the tile once: a cell's upper face, and its lower face only
-  // at a border, counts the non-positive states (as the per-cell form does)
+  // each face of the tile once, x faces on the first half of the tile waves,
+  // y faces on the second (the axis uniform per wave): a cell's upper face,
+  // and its lower face only at a border, counts the non-positive states (as
+  // the per-cell form does)
   double lam_max = 0.0;
-  if (w < 2) {
-    const int j = tid / (FT + 1), i = tid % (FT + 1);
-    if (tid < FT * (FT + 1) && j < my && i <= mx) {
+  constexpr int XW = 64 * (NTW / 2);  // threads on x faces
+  if (tid < XW) {
+    const int j = tid / (TW + 1), i = tid % (TW + 1);
+    if (tid < H * (TW + 1) && j < my && i <= mx) {
       const bool el = rex[3 + j][2 + i], er = rex[3 + j][3 + i];
       lam_max = face_of_states<0>(g, fsl[1 + j][i][1], fsl[1 + j][1 + i][0], el, er, i >= 1 || !el, ctrl, afx[j][i]);
     }
-  } else if (w < 4) {
-    const int t = tid - 128;
-    const int i = t / (FT + 1), j = t % (FT + 1);
-    if (t < FT * (FT + 1) && i < mx && j <= my) {
+  } else if (tid < 2 * XW) {
+    const int t = tid - XW;
+    const int i = t / (H + 1), j = t % (H + 1);
+    if (t < TW * (H + 1) && i < mx && j <= my) {
       const bool el = rex[2 + j][3 + i], er = rex[3 + j][3 + i];
       lam_max = face_of_states<1>(g, fsl[j][1 + i][3], fsl[1 + j][1 + i][2], el, er, j >= 1 || !el, ctrl, afy[i][j]);
     }
@@ -1105,17 +1122,24 @@ __global__ __launch_bounds__(FNT) void k_stage_learned(GridDev g, LearnRun run, 
   }
   if (STAGE == 1 && tid == 0 && run.count != nullptr) {
     const int64_t b = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
-    run.count[2 * b] += (unsigned long long)(scount[0][0] + scount[0][1] + scount[0][2] + scount[0][3]);
-    run.count[2 * b + 1] += (unsigned long long)(scount[1][0] + scount[1][1] + scount[1][2] + scount[1][3]);
+    int nl = 0, nf = 0;
+#pragma unroll
+    for (int v = 0; v < NTW; ++v) {
+      nl += scount[0][v];
+      nf += scount[1][v];
+    }
+    run.count[2 * b] += (unsigned long long)nl;
+    run.count[2 * b + 1] += (unsigned long long)nf;
   }
-  block_max_to<FNT>(ctrl, slot, lam_max, tid);
+  block_max_to<NT>(ctrl, slot, lam_max, tid);
   // stage 3 of a step: the last block advances the time (no separate launch)
   if (STAGE == 3 && mode == MODE_STEP) close_step(ctrl, tid);
 }
 
 int64_t learn_count_slots(const GridDev &g) {
+  // both tilings' block counts (16x4 face-state tiles, 8x8 stage tiles)
   const int64_t a = ((g.n[0] + RTX - 1) / RTX) * ((g.n[1] + RTY - 1) / RTY);
-  const int64_t b = ((g.n[0] + FT - 1) / FT) * ((g.n[1] + FT - 1) / FT);
+  const int64_t b = ((g.n[0] + TW - 1) / TW) * ((g.n[1] + TH - 1) / TH);
   return 2 * (a > b ? a : b);
 }
 
@@ -1129,11 +1153,15 @@ static hipError_t stage_learned_t(const GridDev &g, const LearnRun &run, int mod
   in.R = R;
   in.X = (STAGE == 3) ? W : U;
   double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
-  dim3 grid((unsigned)((g.n[0] + FT - 1) / FT), (unsigned)((g.n[1] + FT - 1) / FT));
+  // 8x8 tiles: 8x4 tiles (two blocks per CU at 128^2) measured slower, 47.0 vs
+  // 42.4 us per learned 128^2 step and 53.7 vs 49.3 per plain 256^2 step
+  dim3 grid((unsigned)((g.n[0] + TW - 1) / TW), (unsigned)((g.n[1] + TH - 1) / TH));
   if (run.net.frags != nullptr)
-    hipLaunchKernelGGL((k_stage_learned<STAGE, true>), grid, dim3(FNT), 0, s, g, run, in, U, out, ctrl, mode);
+    hipLaunchKernelGGL((k_stage_learned<STAGE, true, TH>), grid, dim3(TileShape<TH>::NT), 0, s, g, run, in, U, out,
+                       ctrl, mode);
   else
-    hipLaunchKernelGGL((k_stage_learned<STAGE, false>), grid, dim3(FNT), 0, s, g, run, in, U, out, ctrl, mode);
+    hipLaunchKernelGGL((k_stage_learned<STAGE, false, TH>), grid, dim3(TileShape<TH>::NT), 0, s, g, run, in, U, out,
+                       ctrl, mode);
   return hipGetLastError();
 }
 

@@ -102,6 +102,7 @@ O2_CASES = [
     ("2d o2 periodic x only", (60, 9), [4, 4, 1, 3]),
     ("2d o2 thin", (3, 70), [4, 4, 0, 1]),
     ("2d o2 one row", (130, 1), [0, 2, 1, 3]),
+    ("2d o2 large ragged", (203, 170), [4, 4, 0, 1]),
 ]
 
 

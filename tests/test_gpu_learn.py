@@ -190,7 +190,8 @@ def smooth_state(mmf, n, seed):
 
 
 @pytest.mark.parametrize("n,bc", [([48, 48], [4, 4, 4, 4]), ([50, 37], [0, 1, 3, 2]), ([61, 20], [4, 4, 1, 1]),
-                                  ([7, 13], [4, 4, 0, 3]), ([130, 9], [1, 0, 4, 4]), ([8, 8], [4, 4, 4, 4])])
+                                  ([7, 13], [4, 4, 0, 3]), ([130, 9], [1, 0, 4, 4]), ([8, 8], [4, 4, 4, 4]),
+                                  ([203, 170], [4, 4, 1, 0])])
 def test_fused_stage_equals_split_kernels(mmf, n, bc):
     """The fused stage kernel (8x8 tiles, ring cells recomputed, face states in
     LDS) against the face-state + flux kernels with a network that sets most
