@@ -139,6 +139,23 @@ int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_fina
 int mmf_get_time(mmf_ctx *ctx, double *t, double *dt, int64_t *step, double *stage_eig);
 int mmf_synchronize(mmf_ctx *ctx);
 
+/* ---- output (SolverWriter, solver_writer.cpp:56-119; the writes of
+ *      main.cpp:284-298, 346, 509-540) ------------------------------------- */
+/* VTK XML image data (.vti) of the current state: cell fields solved, velocity,
+ * pressure, temperature, density and residualC/MX/MY/MZ/E, the reference's
+ * names and order.  The conversion runs on the GPU; the residuals are the RHS
+ * of the written state (the reference writes the last stage's).  Returns once
+ * the file is written. */
+int mmf_write_vtk(mmf_ctx *ctx, const char *path);
+/* The same, returning at once: the fields are converted and copied on the
+ * context's stream (later steps queue behind the copy) and a host thread
+ * writes the file meanwhile.  Two writes are in flight at most; a third call
+ * waits for the older one. */
+int mmf_write_vtk_async(mmf_ctx *ctx, const char *path);
+/* Waits for every pending write; reports the first failed one since the last
+ * wait (MMF_ERR_STATE, the path in mmf_last_error). */
+int mmf_wait_writes(mmf_ctx *ctx);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 /* main.cpp:550-561: sum |rho - rho_exact| V over all cells; l2 =
  * sqrt(sum (rho - rho_exact)^2 V) (extension).  rho_exact: host [ncells]. */

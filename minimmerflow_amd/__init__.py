@@ -102,6 +102,9 @@ def lib():
         "mmf_run": ([vp, C.c_int64, i64p, dp], C.c_int),
         "mmf_get_time": ([vp, dp, dp, i64p, dp], C.c_int),
         "mmf_synchronize": ([vp], C.c_int),
+        "mmf_write_vtk": ([vp, C.c_char_p], C.c_int),
+        "mmf_write_vtk_async": ([vp, C.c_char_p], C.c_int),
+        "mmf_wait_writes": ([vp], C.c_int),
         "mmf_error_norms": ([vp, dp, dp, dp], C.c_int),
         "mmf_get_positivity_counters": ([vp, i64p, i64p], C.c_int),
         "mmf_reset_positivity_counters": ([vp], C.c_int),
@@ -423,6 +426,16 @@ class Solver:
 
     def synchronize(self):
         self._c(lib().mmf_synchronize(self.ctx))
+
+    # output (solver_writer.cpp:56-119)
+    def write_vtk(self, path, wait=True):
+        """VTK image data (.vti) of the current state; wait=False returns at once
+        and a host thread writes the file (see wait_writes)."""
+        fn = lib().mmf_write_vtk if wait else lib().mmf_write_vtk_async
+        self._c(fn(self.ctx, os.fsencode(path)))
+
+    def wait_writes(self):
+        self._c(lib().mmf_wait_writes(self.ctx))
 
     # diagnostics
     def error_norms(self, rho_exact):

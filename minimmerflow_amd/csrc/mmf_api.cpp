@@ -153,6 +153,12 @@ int read_ctrl(mmf_ctx *ctx) {
 
 }  // namespace
 
+int mmf_ctx_enqueue_rhs(mmf_ctx *ctx) {
+  hipError_t e = mmf::launch_zero_slot(ctx->dctrl, 3, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "zero slot");
+  return launch_stage_timed(ctx, 1, mmf::MODE_RHS_ONLY);
+}
+
 extern "C" {
 
 const char *mmf_status_string(int s) {
@@ -244,6 +250,7 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
 
 int mmf_destroy(mmf_ctx *ctx) {
   if (!ctx) return MMF_OK;
+  mmf_output_release(ctx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &p : ctx->events) {
     (void)hipEventDestroy(p.first);
@@ -430,9 +437,7 @@ int mmf_get_state(mmf_ctx *ctx, double *U) {
 
 int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
   if (!ctx) return MMF_ERR_INVALID;
-  hipError_t e = mmf::launch_zero_slot(ctx->dctrl, 3, ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "zero slot");
-  int rc = launch_stage_timed(ctx, 1, mmf::MODE_RHS_ONLY);
+  int rc = mmf_ctx_enqueue_rhs(ctx);
   if (rc != MMF_OK) return rc;
   if (rhs) {
     std::vector<double> soa((size_t)(5 * ctx->g.fs));
@@ -465,6 +470,8 @@ int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return MMF_OK;
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -516,6 +523,8 @@ int capture_steps(mmf_ctx *ctx, int slot, int64_t nsteps) {
 }
 
 }  // namespace
+
+extern "C" {
 
 void mmf_drop_step_graphs(mmf_ctx *ctx) {
   for (int i = 0; i < 2; ++i) {

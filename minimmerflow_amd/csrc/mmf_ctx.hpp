@@ -14,6 +14,8 @@
 using mmf::Ctrl;
 using mmf::GridDev;
 
+struct mmf_output;  // VTK writer state (mmf_output.cpp)
+
 struct mmf_ctx {
   mmf_grid_desc desc;
   GridDev g;
@@ -47,7 +49,13 @@ struct mmf_ctx {
   bool use_graphs = true;
   double *dFS = nullptr;
   unsigned long long *dlcount = nullptr;
+  mmf_output *out = nullptr;
 };
+
+// Enqueues the RHS of the current state into dR (stage 1, RHS only).
+int mmf_ctx_enqueue_rhs(mmf_ctx *ctx);
+// Waits for and releases the VTK writer of a context (mmf_output.cpp).
+void mmf_output_release(mmf_ctx *ctx);
 
 // releases the captured step graphs of a context (defined with the C-ABI)
 extern "C" void mmf_drop_step_graphs(mmf_ctx *ctx);

@@ -69,6 +69,25 @@ inline double ctrl_stage_max(const Ctrl &c, int s) {
 
 enum StageMode { MODE_STEP = 0, MODE_RHS_ONLY = 1 };
 
+// Byte offsets of the output fields in one staging buffer (the VTK appended
+// arrays, solver_writer.cpp:56-119): solved (int32), velocity (3 doubles per
+// cell), pressure, temperature, density, the five residual components.
+struct OutputLayout {
+  int64_t solved, vel, p, T, rho, res[5], total;  // five residual components
+};
+
+inline OutputLayout output_layout(int64_t N) {
+  OutputLayout L;
+  L.solved = 0;
+  L.vel = (4 * N + 7) / 8 * 8;
+  L.p = L.vel + 24 * N;
+  L.T = L.p + 8 * N;
+  L.rho = L.T + 8 * N;
+  for (int k = 0; k < 5; ++k) L.res[k] = L.rho + 8 * N * (k + 1);
+  L.total = L.res[4] + 8 * N;
+  return L;
+}
+
 // Kernel launchers (mmf_kernels.hip).  All enqueue on 'stream'.
 hipError_t launch_stage(const GridDev &g, int stage, int mode, const double *U, const double *R, const double *W,
                         double *out_R, double *out_W, double *out_U, Ctrl *ctrl, hipStream_t stream);
@@ -78,6 +97,9 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int sta
 bool march_shape_supported(const GridDev &g, const MarchShape &shape);
 MarchShape default_march_shape(const GridDev &g, int num_cus);
 hipError_t launch_advance(Ctrl *ctrl, hipStream_t stream);
+// Output fields of state U and residual R into 'stage' (layout L).
+hipError_t launch_output_fields(const GridDev &g, const double *U, const double *R, const uint8_t *flags, char *stage,
+                                const OutputLayout &L, hipStream_t stream);
 hipError_t launch_zero_slot(Ctrl *ctrl, int slot, hipStream_t stream);
 hipError_t launch_error_norms(const GridDev &g, const double *U, const double *rho_ex, double *partials,
                               int64_t nblocks, double *out2, hipStream_t stream);

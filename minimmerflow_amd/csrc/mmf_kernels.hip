@@ -244,6 +244,38 @@ hipError_t launch_advance(Ctrl *ctrl, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// output fields (solver_writer.cpp:56-119; main.cpp:511-517 converts every
+// cell with utils::conservative2primitive first): one thread per cell
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_output_fields(GridDev g, const double *U, const double *R,
+                                                       const uint8_t *flags, char *stage, OutputLayout L) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.N) return;
+  const int64_t fs = g.fs;
+  double c[NF], p[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) c[k] = U[k * fs + i];
+  cons2prim(c, p);
+  reinterpret_cast<int32_t *>(stage + L.solved)[i] = (flags == nullptr || (flags[i] & CELL_SOLVED)) ? 1 : 0;
+  double *vel = reinterpret_cast<double *>(stage + L.vel);
+  vel[3 * i] = p[FID_U];
+  vel[3 * i + 1] = p[FID_V];
+  vel[3 * i + 2] = p[FID_W];
+  reinterpret_cast<double *>(stage + L.p)[i] = p[FID_P];
+  reinterpret_cast<double *>(stage + L.T)[i] = p[FID_T];
+  reinterpret_cast<double *>(stage + L.rho)[i] = c[FID_RHO];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) reinterpret_cast<double *>(stage + L.res[k])[i] = R[k * fs + i];
+}
+
+hipError_t launch_output_fields(const GridDev &g, const double *U, const double *R, const uint8_t *flags, char *stage,
+                                const OutputLayout &L, hipStream_t s) {
+  if (g.N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_output_fields, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, g, U, R, flags, stage, L);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_slot(Ctrl *ctrl, int slot, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, ctrl, slot);
   return hipGetLastError();

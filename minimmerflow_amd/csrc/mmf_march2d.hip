@@ -565,7 +565,7 @@ __global__ __launch_bounds__(W2O) void k_march2d_o2(GridDev g, const double *X, 
         Tup TL;
         make_tup(FyL, TL);
         l = face_flux_area(TyU, TL, 1, P2, A, halfA, AFy);
-        if (tup_negative(TyU) | tup_negative(TL)) count_neg(TyU, TL, ctrl);
+        if (tup_negative(TyU) || tup_negative(TL)) count_neg(TyU, TL, ctrl);
       } else if (eb) {  // lower border of the y axis
         Tup TL;
         make_tup(FyL, TL);
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(W2O) void k_march2d_o2(GridDev g, const double *X, 
           Tup TU;
           make_tup(FxU, TU);
           l = face_flux_area(TU, TR, 0, P2, A, halfA, AFx);
-          if (tup_negative(TU) | tup_negative(TR)) count_neg(TU, TR, ctrl);
+          if (tup_negative(TU) || tup_negative(TR)) count_neg(TU, TR, ctrl);
         } else if (exr) {  // this column is outside a non-periodic x border
           l = border_face2(gg, 0, 0, TR, false, AFx, ctrl);
         } else {  // the next column is outside

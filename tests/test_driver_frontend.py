@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -61,12 +62,37 @@ def test_configuration_errors(tmp_path, ptype, bodies, msg):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
 def test_reference_test_directories(tmp_path, case):
+    """The reference's settings unchanged; argv = nCells of the settings and
+    nSaves = 0 (the reference's own command line, main.cpp:103-121), so only
+    the initial, first-step and final states are written instead of one file
+    per step."""
     shutil.copy(os.path.join(HERE, "golden", "reference_settings", case["name"], "settings.xml"), tmp_path)
-    r = run_driver(str(tmp_path))
+    r = run_driver(str(tmp_path), [str(case["nCells"]), "0"])
     assert r.returncode == 0, r.stderr
     got = final_error(r.stdout)
     print(case["name"], got, case["expected"], flush=True)
     assert got == case["expected"]
+    n = case["nCells"]
+    names = sorted(os.listdir(str(tmp_path)))
+    assert names == ["background_%d.0000.vti" % n, "background_%d.0001.vti" % n, "final_background_%d.vti" % n,
+                     "settings.xml"], names
+
+
+@pytest.mark.gpu
+def test_driver_save_schedule(tmp_path):
+    """nSaves = 4 over t in [0, 2] (main.cpp:376, 509-523): a save after the
+    first step whose time passes 0, 0.5, 1 and 1.5, besides the initial and
+    final states; every file holds the reference's ten fields."""
+    from test_gpu_output import FIELDS, read_vti
+    shutil.copy(os.path.join(HERE, "golden", "reference_settings", "2dIsentropicVortex", "settings.xml"), tmp_path)
+    r = run_driver(str(tmp_path), ["32", "4"])
+    assert r.returncode == 0, r.stderr
+    names = sorted(n for n in os.listdir(str(tmp_path)) if n.endswith(".vti"))
+    assert names == ["background_32.%04d.vti" % i for i in range(5)] + ["final_background_32.vti"], names
+    for n in names:
+        meta, arrays = read_vti(str(tmp_path / n))
+        assert meta["names"] == FIELDS and meta["extent"] == [0, 32, 0, 32, 0, 1]
+        assert np.all(arrays["density"] > 0) and np.all(arrays["solved"] == 1)
 
 
 @pytest.mark.gpu
