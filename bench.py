@@ -284,7 +284,7 @@ def run_learned(args, mmf):
     # stage timing of the learned tile kernel (HIP events)
     c = LR.case_params(1, seed=99)[0]
     sv, _ = LR.solver(c, n)
-    sv.set_learned(net, 0.05)
+    sv.set_learned(net, LR.THETA)
     sv.set_time(0.0, 1e30, LR.CFL)
     sv.advance(3)
     sv.enable_timing(True)
