@@ -348,7 +348,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
     ap.add_argument("--ncases", type=int, default=512, help="batch: number of cases")
-    ap.add_argument("--train-cases", type=int, default=16, help="learned: fine training runs")
+    ap.add_argument("--train-cases", type=int, default=32, help="learned: fine training runs")
     ap.add_argument("--eval-cases", type=int, default=4, help="learned: held-out cases")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
