@@ -39,7 +39,8 @@ struct MarchShape {
 // Stage maxima and block counters are spread over EIG_SPLIT cache lines each
 // (block b uses line b % EIG_SPLIT): device-scope atomics on one address
 // from hundreds of blocks serialise at the point of coherence (about 10 ns
-// each), on different lines they proceed in parallel.
+// each), on different lines they proceed in parallel.  Batched cases, whose
+// blocks per case are few, use line 0 only.
 constexpr int EIG_SPLIT = 16;
 constexpr int LINE_U64 = 16;  // 128 bytes
 

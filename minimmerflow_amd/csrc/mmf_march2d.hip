@@ -728,7 +728,7 @@ __global__ __launch_bounds__(W2) void k_batch2d(const BatchCase *cases, const in
     Ctrl *ctrl = bc.ctrl;
     if (!(ctrl->t < ctrl->tmax)) continue;  // this case has reached its end time
     const GridDev &g = bc.g;
-    const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+    const double dt = (STAGE >= 2) ? step_dt_line0(ctrl) : 0.0;
     const int tile = code % bc.ntx;
     const int bz = code / bc.ntx;
     const int nm = (int)g.n[1];
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(W2) void k_batch2d(const BatchCase *cases, const in
     else
       lam = march2d_item<STAGE, false, false>(g, In2<STAGE, false>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds, lfl);
     unsigned long long b = wave_max_bits(lam);
-    if (threadIdx.x == 0) stage_max_put(ctrl, STAGE - 1, b);
+    if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[STAGE - 1][0][0], b);
   }
 }
 
@@ -755,17 +755,15 @@ __global__ void k_batch_advance(const BatchCase *cases, int ncases, unsigned int
   if (i >= ncases) return;
   Ctrl *c = cases[i].ctrl;
   if (c->t < c->tmax) {
-    double dt = step_dt(c);
+    double dt = step_dt_line0(c);
     c->dt = dt;
     c->t += dt;
     c->step += 1;
 #pragma unroll
-    for (int s = 0; s < 3; ++s) c->last_eig[s] = stage_max(c, s);
+    for (int s = 0; s < 3; ++s) c->last_eig[s] = __longlong_as_double((long long)c->eig[s][0][0]);
   }
 #pragma unroll
-  for (int s = 0; s < 3; ++s)
-#pragma unroll
-    for (int k = 0; k < EIG_SPLIT; ++k) c->eig[s][k][0] = 0ull;
+  for (int s = 0; s < 3; ++s) c->eig[s][0][0] = 0ull;
 }
 
 hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,

@@ -39,6 +39,15 @@ __device__ __forceinline__ double step_dt(const Ctrl *c) {
   return dt;
 }
 
+// Batched cases keep their stage maxima on line 0 of their control block
+// alone (a case's blocks are few, so one line does not serialise them).
+__device__ __forceinline__ double step_dt_line0(const Ctrl *c) {
+  const double me = __longlong_as_double((long long)c->eig[0][0][0]);
+  double dt = 0.9 * c->cfl * c->hmin / me;
+  if (c->t + dt > c->tmax) dt = c->tmax - c->t;
+  return dt;
+}
+
 // The end of a step (main.cpp:505-506), by one whole wave: t += dt, the
 // stage maxima kept as the last step's and reset.  Lane 16 s + k reads line
 // k of stage s with an atomic, so a launch whose other blocks formed them may
