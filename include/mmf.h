@@ -88,8 +88,9 @@ enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2, MMF_KER
  * (order 1; order 2 in 2D without bodies).  TILE: the 8x8 LDS tile kernel of
  * 2D order-2 grids without bodies (each face once, the step's time advance
  * in its last stage; also the learned reconstruction's kernel).  AUTO: TILE
- * for 2D order-2 grids without bodies below 2^18 cells, MARCH wherever else
- * it applies (2D order 2 from 2^20 cells), else GATHER.  mmf_set_kernel refuses MARCH or TILE on a grid they do not
+ * for 2D order-2 grids without bodies below 2^18 cells; MARCH where it
+ * applies from 2^18 cells (2D order 1) or 2^20 cells (3D order 1, 2D order
+ * 2); else GATHER.  mmf_set_kernel refuses MARCH or TILE on a grid they do not
  * apply to; bodies set afterwards make them fall back to GATHER.  tile_rows (3D: 4 or 8) and chunk (planes per block along the last axis)
  * shape the marching kernel; <= 0 keeps the default.  Every variant gives
  * bit-identical results. */

@@ -69,12 +69,19 @@ void soa_to_aos(const mmf_ctx *ctx, const std::vector<double> &soa, double *aos)
   }
 }
 
-// Kernel actually used: MARCH where it applies; AUTO takes it as well except
-// for 2D order-2 grids below 2^20 cells, where the per-cell gather kernel is
-// faster (the strip kernel's row march is latency-bound at one wave per SIMD).
+// Kernel actually used: MARCH where it applies; AUTO takes it only from the
+// size where it beats the per-cell gather kernel (measured on MI355X; below
+// it a grid gives the marching kernels too few blocks or rows to march):
+// 2D order 1 from 2^18 cells (64^2 56 vs 28 us per step, 512^2 equal, 1024^2
+// 137 vs 188), 3D order 1 from 2^20 (32^3 273 vs 37 us, 96^3 319 vs 193,
+// 128^3 271 vs 379), 2D order 2 from 2^20 (below it the tile kernel, then the
+// gather kernel).
 bool use_march(const mmf_ctx *ctx) {
   if (ctx->kernel == mmf::KERNEL_GATHER || !mmf::march_shape_supported(ctx->g, ctx->shape)) return false;
-  if (ctx->kernel == mmf::KERNEL_AUTO && ctx->g.order == 2 && ctx->g.N < ((int64_t)1 << 20)) return false;
+  if (ctx->kernel == mmf::KERNEL_AUTO) {
+    const int shift = (ctx->g.dim == 2 && ctx->g.order == 1) ? 18 : 20;
+    if (ctx->g.N < ((int64_t)1 << shift)) return false;
+  }
   return true;
 }
 
