@@ -41,34 +41,6 @@ int hip_fail(mmf_ctx *ctx, hipError_t e, const char *what) {
 
 bool valid_bc(int b) { return b >= MMF_BC_FREE_FLOW && b <= MMF_BC_PERIODIC; }
 
-// AoS host (optionally in raw order) -> SoA host
-void aos_to_soa(const mmf_ctx *ctx, const double *aos, std::vector<double> &soa) {
-  const int64_t N = ctx->g.N, fs = ctx->g.fs;
-  soa.assign((size_t)(5 * fs), 0.0);
-  if (ctx->raw.empty()) {
-    for (int64_t c = 0; c < N; ++c)
-      for (int k = 0; k < 5; ++k) soa[(size_t)(k * fs + c)] = aos[5 * c + k];
-  } else {
-    for (int64_t r = 0; r < N; ++r) {
-      int64_t c = ctx->raw[(size_t)r];
-      for (int k = 0; k < 5; ++k) soa[(size_t)(k * fs + c)] = aos[5 * r + k];
-    }
-  }
-}
-
-void soa_to_aos(const mmf_ctx *ctx, const std::vector<double> &soa, double *aos) {
-  const int64_t N = ctx->g.N, fs = ctx->g.fs;
-  if (ctx->raw.empty()) {
-    for (int64_t c = 0; c < N; ++c)
-      for (int k = 0; k < 5; ++k) aos[5 * c + k] = soa[(size_t)(k * fs + c)];
-  } else {
-    for (int64_t r = 0; r < N; ++r) {
-      int64_t c = ctx->raw[(size_t)r];
-      for (int k = 0; k < 5; ++k) aos[5 * r + k] = soa[(size_t)(k * fs + c)];
-    }
-  }
-}
-
 // Kernel actually used: MARCH where it applies; AUTO takes it only from the
 // size where it beats the per-cell gather kernel (measured on MI355X; below
 // it a grid gives the marching kernels too few blocks or rows to march):
@@ -274,6 +246,7 @@ int mmf_destroy(mmf_ctx *ctx) {
   mmf_drop_step_graphs(ctx);
   if (ctx->dFS) (void)hipFree(ctx->dFS);
   if (ctx->dlcount) (void)hipFree(ctx->dlcount);
+  if (ctx->draw) (void)hipFree(ctx->draw);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMF_OK;
@@ -409,6 +382,11 @@ int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw) {
   if (!ctx) return MMF_ERR_INVALID;
   if (!cell_of_raw) {
     ctx->raw.clear();
+    if (ctx->draw) {
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      HIPCHK(ctx, hipFree(ctx->draw));
+      ctx->draw = nullptr;
+    }
     return MMF_OK;
   }
   if (nraw != ctx->g.N) return fail(ctx, MMF_ERR_INVALID, "raw order size must equal the cell count");
@@ -419,26 +397,33 @@ int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw) {
     seen[(size_t)c] = 1;
   }
   ctx->raw.assign(cell_of_raw, cell_of_raw + nraw);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!ctx->draw) HIPCHK(ctx, hipMalloc(&ctx->draw, sizeof(int64_t) * (size_t)nraw));
+  HIPCHK(ctx, hipMemcpy(ctx->draw, cell_of_raw, sizeof(int64_t) * (size_t)nraw, hipMemcpyHostToDevice));
   return MMF_OK;
 }
 
+// The host AoS buffer crosses PCIe as it is; the transpose to or from the SoA
+// planes runs on the GPU, through the residual buffer R (dead between steps;
+// it holds the 5N doubles of an AoS copy since fs >= N).
 int mmf_set_state(mmf_ctx *ctx, const double *U) {
   if (!ctx || !U) return MMF_ERR_INVALID;
-  std::vector<double> soa;
-  aos_to_soa(ctx, U, soa);
+  const int64_t N = ctx->g.N, fs = ctx->g.fs;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  HIPCHK(ctx, hipMemcpy(ctx->dU, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->dR, U, sizeof(double) * 5 * (size_t)N, hipMemcpyHostToDevice));
+  HIPCHK(ctx, mmf::launch_aos_to_soa(ctx->dR, ctx->draw, N, fs, ctx->dU, ctx->stream));
   // the stage buffer starts as a copy so cells never solved hold finite values
-  HIPCHK(ctx, hipMemcpy(ctx->dW, ctx->dU, sizeof(double) * soa.size(), hipMemcpyDeviceToDevice));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->dW, ctx->dU, sizeof(double) * 5 * (size_t)fs, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return MMF_OK;
 }
 
 int mmf_get_state(mmf_ctx *ctx, double *U) {
   if (!ctx || !U) return MMF_ERR_INVALID;
-  std::vector<double> soa((size_t)(5 * ctx->g.fs));
+  const int64_t N = ctx->g.N, fs = ctx->g.fs;
+  HIPCHK(ctx, mmf::launch_soa_to_aos(ctx->dU, ctx->draw, N, fs, ctx->dR, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  HIPCHK(ctx, hipMemcpy(soa.data(), ctx->dU, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
-  soa_to_aos(ctx, soa, U);
+  HIPCHK(ctx, hipMemcpy(U, ctx->dR, sizeof(double) * 5 * (size_t)N, hipMemcpyDeviceToHost));
   return MMF_OK;
 }
 
@@ -447,10 +432,13 @@ int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
   int rc = mmf_ctx_enqueue_rhs(ctx);
   if (rc != MMF_OK) return rc;
   if (rhs) {
-    std::vector<double> soa((size_t)(5 * ctx->g.fs));
-    HIPCHK(ctx, hipMemcpyAsync(soa.data(), ctx->dR, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, ctx->stream));
+    // AoS residual through W (dead between steps), which is then restored to
+    // the copy of U it holds after mmf_set_state
+    const int64_t N = ctx->g.N, fs = ctx->g.fs;
+    HIPCHK(ctx, mmf::launch_soa_to_aos(ctx->dR, ctx->draw, N, fs, ctx->dW, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    soa_to_aos(ctx, soa, rhs);
+    HIPCHK(ctx, hipMemcpy(rhs, ctx->dW, sizeof(double) * 5 * (size_t)N, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dW, ctx->dU, sizeof(double) * 5 * (size_t)fs, hipMemcpyDeviceToDevice, ctx->stream));
   }
   int r2 = read_ctrl(ctx);
   if (r2 != MMF_OK) return r2;

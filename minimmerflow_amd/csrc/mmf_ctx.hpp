@@ -30,6 +30,7 @@ struct mmf_ctx {
   int64_t nerr_blocks = 0;
   double *derr = nullptr;
   std::vector<int64_t> raw;  // cell_of_raw, empty when identity
+  int64_t *draw = nullptr;   // the same on the device (null when identity)
   int kernel = mmf::KERNEL_AUTO;
   mmf::MarchShape shape{8, 32};
   std::string err;

@@ -98,6 +98,11 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int sta
 bool march_shape_supported(const GridDev &g, const MarchShape &shape);
 MarchShape default_march_shape(const GridDev &g, int num_cus);
 hipError_t launch_advance(Ctrl *ctrl, hipStream_t stream);
+// AoS rows (raw order via cell_of_raw, identity when null) <-> SoA planes.
+hipError_t launch_aos_to_soa(const double *aos, const int64_t *cell_of_raw, int64_t N, int64_t fs, double *soa,
+                             hipStream_t stream);
+hipError_t launch_soa_to_aos(const double *soa, const int64_t *cell_of_raw, int64_t N, int64_t fs, double *aos,
+                             hipStream_t stream);
 // Output fields of state U and residual R into 'stage' (layout L).
 hipError_t launch_output_fields(const GridDev &g, const double *U, const double *R, const uint8_t *flags, char *stage,
                                 const OutputLayout &L, hipStream_t stream);

@@ -276,6 +276,43 @@ hipError_t launch_output_fields(const GridDev &g, const double *U, const double 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// host layout <-> device layout (storage.hpp:31-41): AoS rows of 5 doubles in
+// raw order (row r holds cell cell_of_raw[r]; identity when null) and the
+// SoA planes of the state
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_aos_to_soa(const double *aos, const int64_t *cell_of_raw, int64_t N,
+                                                    int64_t fs, double *soa) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= N) return;
+  const int64_t c = cell_of_raw ? cell_of_raw[r] : r;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) soa[k * fs + c] = aos[NF * r + k];
+}
+
+__global__ __launch_bounds__(256) void k_soa_to_aos(const double *soa, const int64_t *cell_of_raw, int64_t N,
+                                                    int64_t fs, double *aos) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= N) return;
+  const int64_t c = cell_of_raw ? cell_of_raw[r] : r;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) aos[NF * r + k] = soa[k * fs + c];
+}
+
+hipError_t launch_aos_to_soa(const double *aos, const int64_t *cell_of_raw, int64_t N, int64_t fs, double *soa,
+                             hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, aos, cell_of_raw, N, fs, soa);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(const double *soa, const int64_t *cell_of_raw, int64_t N, int64_t fs, double *aos,
+                             hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, soa, cell_of_raw, N, fs, aos);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_slot(Ctrl *ctrl, int slot, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_slot, dim3(1), dim3(64), 0, s, ctrl, slot);
   return hipGetLastError();

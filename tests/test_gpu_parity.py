@@ -176,6 +176,11 @@ def test_raw_order_permutation(mmf, orc):
     gs.set_state(U[perm])
     Rg, _ = gs.compute_rhs()
     assert np.array_equal(Rg, Ro[perm])
+    assert np.array_equal(gs.get_state(), U[perm])  # the transposes run on the device
+    gs.set_raw_order(None)
+    assert np.array_equal(gs.get_state(), U)
+    Rg, _ = gs.compute_rhs()
+    assert np.array_equal(Rg, Ro)
 
 
 # ---- time loop -----------------------------------------------------------------------
