@@ -1,19 +1,20 @@
 // mmf_march.hip -- marching stage kernel for order-1 reconstruction (gfx950).
 //
-// A block owns a tile of 64 (x, one wave per row) by TY (y) cells and marches
-// along the last axis (z in 3D, y in 2D) over a chunk of planes.  Per plane:
-//   1. every thread turns its cell's state (loaded one plane ahead, so the
-//      HBM latency hides behind the previous plane's work) into the cell's
-//      flux tuple -- cons2prim, sound speed, total enthalpy: every division
-//      and the sqrt of the reference's per-face evaluation, done once per
-//      cell -- and stores it in LDS; the first threads do the same for the
-//      tile's one-cell halo ring;
-//   2. the face between this plane and the previous one uses the previous
-//      tuple kept in registers, so each march-axis face is evaluated once; it
-//      completes the previous cell, whose RK stage update is written;
-//   3. after one barrier, the x and y faces of each cell read the neighbour
-//      tuples from LDS.  Tuple planes are double-buffered in LDS (TY <= 8), so
-//      one barrier per plane suffices.
+// A block owns a tile of 64 (x, one wave per row) by TY (y) cells of a 3D grid
+// and marches along z over a chunk of planes.  Each thread turns its cell's
+// state into the cell's flux tuple -- cons2prim, sound speed, total enthalpy:
+// every division and the sqrt of the reference's per-face evaluation, done
+// once per cell -- and the first threads do the same for the tile's one-cell
+// halo ring.  Tuple planes are double-buffered in LDS, and one pass over plane
+// m, between two barriers, does all of:
+//   - the tuples of plane m + 1 (its state loaded during plane m - 1, so the
+//     HBM latency hides behind a whole plane of work) into the other buffer,
+//     whose last readers finished before the barrier that ended plane m - 1;
+//   - the x and y faces of the cell in plane m, from the neighbour tuples in
+//     LDS, evaluated while plane m + 1's division chains are in flight;
+//   - the z face between planes m and m + 1 from the two tuples in registers
+//     (each z face once), which completes the cell: its RK stage update is
+//     written in the same pass.
 // Residual contributions are accumulated per cell in the reference's face
 // order (x lower, x upper, y lower, y upper, z lower, z upper), so the result
 // is bit-identical to the gather kernel and to the oracle; no float atomics.
@@ -153,10 +154,9 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   constexpr int M = DIM - 1;  // march axis
   constexpr int RW = TX + 2;
   constexpr int PL = (TY + 2) * RW;
-  // two tuple planes (one barrier per plane) where they fit in LDS
-  constexpr int NBUF = (TY <= 8) ? 2 : 1;
-  __shared__ double lds[NBUF * NTUP * PL];
-  __shared__ uint8_t lfl[MASK ? NBUF * PL : 1];
+  static_assert(TY <= 8, "two tuple planes must fit in LDS");
+  __shared__ double lds[2 * NTUP * PL];
+  __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
 
   const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
@@ -236,7 +236,11 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const bool has_ym = DIM == 3 && valid && (gy > 0 || g.periodic[1]);
   const bool has_yp = DIM == 3 && valid && (gy + 1 < ny || g.periodic[1]);
   const bool wrap_y = DIM == 3 && g.periodic[1] && gy == ny - 1;
-  const bool top_wrap_here = g.periodic[M] && m1 == nm;  // this chunk holds the periodic top cell
+  // the plane above the chunk's last one: the next chunk's first, or plane 0
+  // across a periodic top (then the chunk's last cell is the periodic top cell)
+  const bool top_wrap_here = g.periodic[M] && m1 == nm;
+  const bool have_beyond = (m1 < nm) || g.periodic[M];
+  const uint32_t beyond = (m1 < nm) ? (uint32_t)m1 : 0u;
   // P2: A and V are powers of two (compile-time, so no select between the forms)
   const bool pow2A = P2;
   GridDev gp = g;
@@ -245,129 +249,135 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
-  Tup Tp;  // previous plane's tuple (this column)
-  uint8_t fprev = CELL_FLUID | CELL_SOLVED;
-  bool bad_prev = false;  // previous tuple has a negative p or rho
-  double part[NF];        // residual of the previous cell so far
+  Tup Tc;  // tuple of this column's cell in plane m
+  uint8_t fc = CELL_FLUID | CELL_SOLVED;
+  bool bad_c = false;   // Tc has a negative p or rho
+  double AFlo[NF];      // A F of the face below plane m
+  double Ucur[NF];      // U of the cell in plane m (stage 2: carried from its load)
 #pragma unroll
-  for (int k = 0; k < NF; ++k) part[k] = 0.0;
+  for (int k = 0; k < NF; ++k) AFlo[k] = Ucur[k] = 0.0;
 
-  // prologue: the plane below the chunk
+  // ---- prologue: plane m0 into LDS buffer 0, and the face below it ----
   {
-    int mprev = m0 - 1;
-    bool have_prev = true;
-    if (mprev < 0) {
-      if (g.periodic[M]) mprev += nm;
-      else have_prev = false;
-    }
-    if (valid && have_prev) {
+    const uint32_t idx0 = col + (uint32_t)m0 * sm;
+    if (valid) {
+      Prefetch<STAGE> p0;
       double X[NF];
-      src.load(gp, dt, col + (uint32_t)mprev * sm, X);
-      make_tup(X, Tp);
-      fprev = cell_flags<MASK>(g, col + (uint32_t)mprev * sm);
-      bad_prev = tup_negative(Tp);
+      p0.issue(src, idx0);
+      p0.state(gp, dt, X);
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = p0.a[k];
+      }
+      make_tup(X, Tc);
+      bad_c = tup_negative(Tc);
+      fc = cell_flags<MASK>(g, idx0);
+      tup_store<PL>(lds, myslot, Tc);
+      if (MASK) lfl[myslot] = fc;
+    }
+    if (hslot >= 0) {
+      double Xh[NF];
+      Tup Th;
+      src.load(gp, dt, hcol + (uint32_t)m0 * sm, Xh);
+      make_tup(Xh, Th);
+      tup_store<PL>(lds, hslot, Th);
+      if (MASK) lfl[hslot] = g.flags[hcol + (uint32_t)m0 * sm];
+    }
+    if (valid) {
+      int mprev = m0 - 1;
+      bool have_prev = true;
+      if (mprev < 0) {
+        if (g.periodic[M]) mprev += nm;
+        else have_prev = false;
+      }
+      if (!have_prev) {  // lower border of the march axis
+        if (fc & CELL_SOLVED) {
+          const double l = virtual_face(gp, false, 2 * M, M, Tc, false, AFlo, ctrl, true);
+          lam_max = (l < lam_max) ? lam_max : l;
+        }
+      } else {
+        double Xp[NF];
+        Tup Tp;
+        const uint32_t pidx = col + (uint32_t)mprev * sm;
+        src.load(gp, dt, pidx, Xp);
+        make_tup(Xp, Tp);
+        const uint8_t fp = cell_flags<MASK>(g, pidx);
+        // counted here: the block below does not count its chunk's top face
+        if ((fp | fc) & CELL_SOLVED) {
+          double l;
+          if (MASK && ((fp ^ fc) & CELL_FLUID)) {
+            if (fc & CELL_FLUID) l = virtual_face(gp, true, 2 * M, M, Tc, false, AFlo, ctrl, true);
+            else l = virtual_face(gp, true, 2 * M + 1, M, Tp, true, AFlo, ctrl, true);
+          } else {
+            l = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFlo);
+            if (tup_negative(Tp) | bad_c) count_neg(Tp, Tc, ctrl);
+          }
+          lam_max = (l < lam_max) ? lam_max : l;
+        }
+      }
     }
   }
-  const bool have_prev0 = (m0 > 0) || g.periodic[M];
 
+  // loads of plane m0 + 1 (or of the plane beyond a one-plane chunk)
   Prefetch<STAGE> pf, ph;
-  if (valid) pf.issue(src, col + (uint32_t)m0 * sm);
-  if (hslot >= 0) ph.issue(src, hcol + (uint32_t)m0 * sm);
-
-  // U of the previous / current cell for the RK update (stages 2 and 3)
-  double Uf[NF], Ucur[NF];
-#pragma unroll
-  for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k] = 0.0;
+  if (valid && (m0 + 1 < m1 || have_beyond)) pf.issue(src, col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm);
+  if (hslot >= 0 && m0 + 1 < m1) ph.issue(src, hcol + (uint32_t)(m0 + 1) * sm);
+  __syncthreads();
 
   int buf = 0;
   for (int m = m0; m < m1; ++m) {
     const uint32_t idx = col + (uint32_t)m * sm;
-    double *L = lds + buf * (NTUP * PL);
-    uint8_t *LF = lfl + (MASK ? buf * PL : 0);
-    Tup Tc;
-    uint8_t fc = CELL_FLUID | CELL_SOLVED;
-    bool bad_c = false;
-    // take this plane's inputs and put the next plane's loads in flight at
-    // once: they then have a whole plane of work to arrive
-    double X[NF], Xh[NF];
-    if (valid) {
+    const double *L = lds + buf * (NTUP * PL);
+    const uint8_t *LF = lfl + (MASK ? buf * PL : 0);
+    double *Ln = lds + (buf ^ 1) * (NTUP * PL);
+    uint8_t *LFn = lfl + (MASK ? (buf ^ 1) * PL : 0);
+    const bool in_chunk = m + 1 < m1;                // plane m + 1 is this chunk's
+    const bool has_next = in_chunk || have_beyond;  // a cell above this one
+    const uint32_t nplane = in_chunk ? (uint32_t)(m + 1) : beyond;
+    if (STAGE == 3 && valid) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
+    }
+
+    // ---- plane m + 1: tuples into the other LDS buffer, whose last readers
+    // (plane m - 1's faces) finished before the barrier that ended plane m - 1;
+    // built while this plane's faces are evaluated ----
+    Tup Tn;
+    uint8_t fn = CELL_FLUID | CELL_SOLVED;
+    bool bad_n = false;
+    double Unext[NF];
+    if (valid && has_next) {
+      double X[NF];
       pf.state(gp, dt, X);
       if (STAGE == 2) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) Ucur[k] = pf.a[k];
-      } else if (STAGE == 3) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
+        for (int k = 0; k < NF; ++k) Unext[k] = pf.a[k];
+      }
+      make_tup(X, Tn);
+      bad_n = tup_negative(Tn);
+      fn = cell_flags<MASK>(g, col + nplane * sm);
+      if (in_chunk) {
+        tup_store<PL>(Ln, myslot, Tn);
+        if (MASK) LFn[myslot] = fn;
       }
     }
-    if (hslot >= 0) ph.state(gp, dt, Xh);
-    if (m + 1 < m1) {
-      if (valid) pf.issue(src, idx + sm);
-      if (hslot >= 0) ph.issue(src, hcol + (uint32_t)(m + 1) * sm);
-    }
-    if (valid) {
-      make_tup(X, Tc);
-      bad_c = tup_negative(Tc);
-      fc = cell_flags<MASK>(g, idx);
-      tup_store<PL>(L, myslot, Tc);
-      if (MASK) LF[myslot] = fc;
-    }
-    if (hslot >= 0) {
+    if (hslot >= 0 && in_chunk) {
+      double Xh[NF];
       Tup Th;
+      ph.state(gp, dt, Xh);
       make_tup(Xh, Th);
-      tup_store<PL>(L, hslot, Th);
-      if (MASK) LF[hslot] = g.flags[hcol + (uint32_t)m * sm];
+      tup_store<PL>(Ln, hslot, Th);
+      if (MASK) LFn[hslot] = g.flags[hcol + (uint32_t)(m + 1) * sm];
     }
-
-    // ---- face between plane m-1 and plane m (evaluated once) ----
-    double AFz[NF];
-    bool zact = false;
-    if (valid) {
-      const bool hl = (m > m0) || have_prev0;
-      if (!hl) {  // lower border of the march axis
-        zact = fc & CELL_SOLVED;
-        if (zact) {
-          double lz = virtual_face(gp, false, 2 * M, M, Tc, false, AFz, ctrl, true);
-          lam_max = (lz < lam_max) ? lam_max : lz;
-        }
-      } else {
-        zact = (fprev | fc) & CELL_SOLVED;
-        if (zact) {
-          double lz;
-          if (MASK && ((fprev ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) lz = virtual_face(gp, true, 2 * M, M, Tc, false, AFz, ctrl, true);
-            else lz = virtual_face(gp, true, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
-          } else {
-            lz = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFz);
-            if (bad_prev | bad_c) count_neg(Tp, Tc, ctrl);
-          }
-          lam_max = (lz < lam_max) ? lam_max : lz;
-        }
-      }
-
-      // ---- the previous cell is complete ----
-      if (m > m0) {
-        const uint32_t pidx = idx - sm;
-        if (fprev & CELL_SOLVED) {
-#pragma unroll
-          for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(gp, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
-        } else if (STAGE != 3) {
-#pragma unroll
-          for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
-        }
-      }
-      // this cell's U, for its own update one plane later
-      if (STAGE >= 2) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k];
-      }
-    }
-    __syncthreads();
+    // the loads of plane m + 2 (a whole plane of work to arrive)
+    if (valid && (m + 2 < m1 || (m + 2 == m1 && have_beyond)))
+      pf.issue(src, col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm);
+    if (hslot >= 0 && m + 2 < m1) ph.issue(src, hcol + (uint32_t)(m + 2) * sm);
 
     if (valid) {
       // ---- in-plane faces of this cell, reference order ----
+      double acc[NF] = {0., 0., 0., 0., 0.};  // euler.cpp:142-148
       if (fc & CELL_SOLVED) {
-        double acc[NF] = {0., 0., 0., 0., 0.};  // euler.cpp:142-148
 #pragma unroll
         for (int a = 0; a < DIM - 1; ++a) {
           const bool hm = (a == 0) ? has_xm : has_ym;
@@ -413,82 +423,57 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
             for (int k = 0; k < NF; ++k) { acc[k] += AFm[k]; acc[k] -= AFp[k]; }
           }
         }
-        if (top_wrap_here && m == nm - 1) {
-          // periodic top cell: its upper face (the wrap face) precedes its
-          // lower face in the reference's face list; finish the cell here
-          double X0[NF], AFw[NF];
-          Tup T0;
-          src.load(gp, dt, col, X0);
-          make_tup(X0, T0);
-          const uint8_t f0 = cell_flags<MASK>(g, col);
-          double lw;
-          if (MASK && ((f0 ^ fc) & CELL_FLUID)) {
-            lw = virtual_face(gp, true, 2 * M + 1, M, Tc, true, AFw, ctrl, false);
-          } else {
-            lw = face_flux_area(Tc, T0, M, pow2A, A, halfA, AFw);
-          }
-          lam_max = (lw < lam_max) ? lam_max : lw;
-#pragma unroll
-          for (int k = 0; k < NF; ++k) {
-            double rhs = (acc[k] - AFw[k]) + (zact ? AFz[k] : 0.0);
-            outf.put(k, idx, rk_update<STAGE>(gp, Uf[k], Tc.c[k], rhs, dt));
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < NF; ++k) part[k] = acc[k] + (zact ? AFz[k] : 0.0);
-        }
       }
-      Tp = Tc;
-      fprev = fc;
-      bad_prev = bad_c;
-    }
-    if (NBUF == 1) __syncthreads();
-    buf ^= (NBUF - 1);
-  }
 
-  // ---- epilogue: the upper face of the chunk's last cell ----
-  if (valid) {
-    const uint32_t pidx = col + (uint32_t)(m1 - 1) * sm;
-    if (top_wrap_here) {
-      if (!(fprev & CELL_SOLVED) && STAGE != 3) {  // solved top cells were finished in the loop
-#pragma unroll
-        for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
-      }
-    } else {
-      const bool have_next = m1 < nm;
-      double AFz[NF];
-      bool act;
-      double lz = 0.0;
-      if (have_next) {
-        double X[NF];
-        Tup Tn;
-        const uint32_t nidx = col + (uint32_t)m1 * sm;
-        src.load(gp, dt, nidx, X);
-        make_tup(X, Tn);
-        const uint8_t fn = cell_flags<MASK>(g, nidx);
-        act = (fprev | fn) & CELL_SOLVED;
-        // counted by the block of the next chunk
-        if (act) {
-          if (MASK && ((fprev ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) lz = virtual_face(gp, true, 2 * M, M, Tn, false, AFz, ctrl, false);
-            else lz = virtual_face(gp, true, 2 * M + 1, M, Tp, true, AFz, ctrl, false);
+      // ---- the face above this cell (each march-axis face once) ----
+      double AFhi[NF] = {0., 0., 0., 0., 0.};
+      if (has_next) {
+        if ((fc | fn) & CELL_SOLVED) {
+          // the chunk's top face and the periodic wrap face are counted by
+          // the block that holds the plane above
+          double l;
+          if (MASK && ((fc ^ fn) & CELL_FLUID)) {
+            if (fn & CELL_FLUID) l = virtual_face(gp, true, 2 * M, M, Tn, false, AFhi, ctrl, in_chunk);
+            else l = virtual_face(gp, true, 2 * M + 1, M, Tc, true, AFhi, ctrl, in_chunk);
           } else {
-            lz = face_flux_area(Tp, Tn, M, pow2A, A, halfA, AFz);
+            l = face_flux_area(Tc, Tn, M, pow2A, A, halfA, AFhi);
+            if (in_chunk && (bad_c | bad_n)) count_neg(Tc, Tn, ctrl);
           }
+          lam_max = (l < lam_max) ? lam_max : l;
         }
-      } else {  // upper border of the march axis
-        act = fprev & CELL_SOLVED;
-        if (act) lz = virtual_face(gp, false, 2 * M + 1, M, Tp, true, AFz, ctrl, true);
+      } else if (fc & CELL_SOLVED) {  // upper border of the march axis
+        const double l = virtual_face(gp, false, 2 * M + 1, M, Tc, true, AFhi, ctrl, true);
+        lam_max = (l < lam_max) ? lam_max : l;
       }
-      if (act) lam_max = (lz < lam_max) ? lam_max : lz;
-      if (fprev & CELL_SOLVED) {
+
+      // ---- this cell is complete: its RK stage update ----
+      if (fc & CELL_SOLVED) {
+        // the periodic top cell: its upper face (the wrap face) precedes its
+        // lower face in the reference's face list
+        const bool wrap_top = top_wrap_here && m == nm - 1;
 #pragma unroll
-        for (int k = 0; k < NF; ++k) outf.put(k, pidx, rk_update<STAGE>(gp, Uf[k], Tp.c[k], part[k] - AFz[k], dt));
+        for (int k = 0; k < NF; ++k) {
+          const double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
+          outf.put(k, idx, rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt));
+        }
       } else if (STAGE != 3) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) outf.put(k, pidx, (STAGE == 1) ? 0.0 : Uf[k]);
+        for (int k = 0; k < NF; ++k) outf.put(k, idx, (STAGE == 1) ? 0.0 : Ucur[k]);
+      }
+
+      // the next cell's lower face, tuple and U
+#pragma unroll
+      for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
+      Tc = Tn;
+      fc = fn;
+      bad_c = bad_n;
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = Unext[k];
       }
     }
+    __syncthreads();
+    buf ^= 1;
   }
   block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
 }
