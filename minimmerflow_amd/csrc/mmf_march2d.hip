@@ -3,12 +3,12 @@
 // One wave owns an item: a strip of 62 cells along x plus one halo cell on
 // each side (lanes 0 and 63; x0 = 62 * tile), marching along y over a chunk
 // of rows.  Every lane builds exactly one flux tuple per row (interior or
-// halo), so no lane does a second pass.  Per row:
-//   1. take the row's inputs (loaded one row ahead) and build the tuple; put
-//      the next row's loads in flight; store the tuple in LDS (two buffers);
-//   2. evaluate the y face to the previous row from registers (each y face
-//      once), which completes the previous cell: write its RK stage update;
-//   3. after a wave barrier, evaluate the two x faces from LDS neighbours.
+// halo), so no lane does a second pass.  Tuple rows are double-buffered in
+// LDS; one pass over row m does all of:
+//   - the tuples of row m + 1 (loaded during row m - 1) into the other buffer;
+//   - the two x faces of the cell in row m from the LDS neighbours;
+//   - the y face between rows m and m + 1 from the two tuples in registers
+//     (each y face once), which completes the cell: its RK stage update.
 // Each cell's residual is summed in the reference's face order (x lower, x
 // upper, y lower, y upper), bit-identical to the gather kernel and the oracle.
 //
@@ -134,110 +134,125 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
   const uint32_t sm = (uint32_t)nx;
   const bool has_xm = valid && (gx > 0 || g.periodic[0]);
   const bool has_xp = valid && (gx + 1 < nx || g.periodic[0]);
+  // the row above the chunk's last one: the next chunk's first, or row 0
+  // across a periodic top (then the chunk's last cell is the periodic top cell)
   const bool top_wrap_here = g.periodic[1] && m1 == nm;
+  const bool have_beyond = (m1 < nm) || g.periodic[1];
+  const uint32_t beyond = (m1 < nm) ? (uint32_t)m1 : 0u;
   const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
-  Tup Tp;
-  uint8_t fprev = CELL_FLUID | CELL_SOLVED;
-  bool bad_prev = false;
-  double part[NF];
+  Tup Tc;  // tuple of this lane's cell in row m
+  uint8_t fc = CELL_FLUID | CELL_SOLVED;
+  bool bad_c = false;
+  double AFlo[NF], Ucur[NF];  // A F of the face below row m; U of the cell in row m
 #pragma unroll
-  for (int k = 0; k < NF; ++k) part[k] = 0.0;
+  for (int k = 0; k < NF; ++k) AFlo[k] = Ucur[k] = 0.0;
 
+  // ---- prologue: row m0 into LDS buffer 0, and the face below it ----
   {
-    int mprev = m0 - 1;
-    bool have_prev = true;
-    if (mprev < 0) {
-      if (g.periodic[1]) mprev += nm;
-      else have_prev = false;
-    }
-    if (valid && have_prev) {
-      double X[NF];
-      in.state(g, dt, col + (uint32_t)mprev * sm, X);
-      make_tup(X, Tp);
-      fprev = cell_flags<MASK>(g, col + (uint32_t)mprev * sm);
-      bad_prev = tup_negative(Tp);
-    }
-  }
-  const bool have_prev0 = (m0 > 0) || g.periodic[1];
-
-  Pf2<STAGE> pf;
-  if (has_tuple) pf.issue(in, col + (uint32_t)m0 * sm);
-  double Uf[NF], Ucur[NF];
-#pragma unroll
-  for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k] = 0.0;
-
-  int buf = 0;
-  for (int m = m0; m < m1; ++m) {
-    const uint32_t idx = col + (uint32_t)m * sm;
-    double *L = ldsb + buf * (NTUP * W2);
-    uint8_t *LF = lflb + (MASK ? buf * W2 : 0);
-    Tup Tc;
-    uint8_t fc = CELL_FLUID | CELL_SOLVED;
-    bool bad_c = false;
-    double X[NF];
+    const uint32_t idx0 = col + (uint32_t)m0 * sm;
     if (has_tuple) {
-      pf.state(g, dt, X);
-      if (valid && STAGE == 2) {
+      Pf2<STAGE> p0;
+      double X[NF];
+      p0.issue(in, idx0);
+      p0.state(g, dt, X);
+      if (STAGE == 2) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) Ucur[k] = pf.a[k];
-      } else if (valid && STAGE == 3) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Ucur[k] = ld(Ub, fs, k, idx);
+        for (int k = 0; k < NF; ++k) Ucur[k] = p0.a[k];
       }
-      if (m + 1 < m1) pf.issue(in, idx + sm);
       make_tup(X, Tc);
       bad_c = tup_negative(Tc);
-      fc = cell_flags<MASK>(g, idx);
-      tstore(L, lane, Tc);
-      if (MASK) LF[lane] = fc;
+      fc = cell_flags<MASK>(g, idx0);
+      tstore(ldsb, lane, Tc);
+      if (MASK) lflb[lane] = fc;
     }
-
-    double AFy[NF];
-    bool yact = false;
     if (valid) {
-      const bool hl = (m > m0) || have_prev0;
-      if (!hl) {
-        yact = fc & CELL_SOLVED;
-        if (yact) {
-          double l = vface(g, false, 2, 1, Tc, false, AFy, ctrl, true);
+      int mprev = m0 - 1;
+      bool have_prev = true;
+      if (mprev < 0) {
+        if (g.periodic[1]) mprev += nm;
+        else have_prev = false;
+      }
+      if (!have_prev) {  // lower border
+        if (fc & CELL_SOLVED) {
+          const double l = vface(g, false, 2, 1, Tc, false, AFlo, ctrl, true);
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else {
-        yact = (fprev | fc) & CELL_SOLVED;
-        if (yact) {
+        double Xp[NF];
+        Tup Tp;
+        const uint32_t pidx = col + (uint32_t)mprev * sm;
+        in.state(g, dt, pidx, Xp);
+        make_tup(Xp, Tp);
+        const uint8_t fp = cell_flags<MASK>(g, pidx);
+        // counted here: the item below does not count its chunk's top face
+        if ((fp | fc) & CELL_SOLVED) {
           double l;
-          if (MASK && ((fprev ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) l = vface(g, true, 2, 1, Tc, false, AFy, ctrl, true);
-            else l = vface(g, true, 3, 1, Tp, true, AFy, ctrl, true);
+          if (MASK && ((fp ^ fc) & CELL_FLUID)) {
+            if (fc & CELL_FLUID) l = vface(g, true, 2, 1, Tc, false, AFlo, ctrl, true);
+            else l = vface(g, true, 3, 1, Tp, true, AFlo, ctrl, true);
           } else {
-            l = face_flux_area(Tp, Tc, 1, P2, A, halfA, AFy);
-            if (bad_prev | bad_c) count_neg(Tp, Tc, ctrl);
+            l = face_flux_area(Tp, Tc, 1, P2, A, halfA, AFlo);
+            if (tup_negative(Tp) | bad_c) count_neg(Tp, Tc, ctrl);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
       }
-      if (m > m0) {
-        const uint32_t pidx = idx - sm;
-        if (fprev & CELL_SOLVED) {
+    }
+  }
+
+  // loads of row m0 + 1 (halo lanes only inside the chunk)
+  Pf2<STAGE> pf;
+  if ((valid && (m0 + 1 < m1 || have_beyond)) || (halo && m0 + 1 < m1))
+    pf.issue(in, col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm);
+  __syncthreads();
+
+  int buf = 0;
+  for (int m = m0; m < m1; ++m) {
+    const uint32_t idx = col + (uint32_t)m * sm;
+    const double *L = ldsb + buf * (NTUP * W2);
+    const uint8_t *LF = lflb + (MASK ? buf * W2 : 0);
+    double *Ln = ldsb + (buf ^ 1) * (NTUP * W2);
+    uint8_t *LFn = lflb + (MASK ? (buf ^ 1) * W2 : 0);
+    const bool in_chunk = m + 1 < m1;                // row m + 1 is this chunk's
+    const bool has_next = in_chunk || have_beyond;  // a cell above this one
+    const uint32_t nrow = in_chunk ? (uint32_t)(m + 1) : beyond;
+    if (STAGE == 3 && valid) {
 #pragma unroll
-          for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFy[k], dt);
-        } else if (STAGE != 3) {
+      for (int k = 0; k < NF; ++k) Ucur[k] = ld(Ub, fs, k, idx);
+    }
+
+    // ---- row m + 1: tuples into the other LDS buffer (its last readers, row
+    // m - 1's x faces, finished before the barrier that ended row m - 1) ----
+    Tup Tn;
+    uint8_t fn = CELL_FLUID | CELL_SOLVED;
+    bool bad_n = false;
+    double Unext[NF];
+    const bool build = (valid && has_next) || (halo && in_chunk);
+    if (build) {
+      double X[NF];
+      pf.state(g, dt, X);
+      if (STAGE == 2) {
 #pragma unroll
-          for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
-        }
+        for (int k = 0; k < NF; ++k) Unext[k] = pf.a[k];
       }
-      if (STAGE >= 2) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) Uf[k] = Ucur[k];
+      make_tup(X, Tn);
+      bad_n = tup_negative(Tn);
+      fn = cell_flags<MASK>(g, col + nrow * sm);
+      if (in_chunk) {
+        tstore(Ln, lane, Tn);
+        if (MASK) LFn[lane] = fn;
       }
     }
-    __syncthreads();
+    // the loads of row m + 2
+    if ((valid && (m + 2 < m1 || (m + 2 == m1 && have_beyond))) || (halo && m + 2 < m1))
+      pf.issue(in, col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm);
 
     if (valid) {
+      // ---- x faces of this cell from the LDS neighbours, reference order ----
+      double acc[NF] = {0., 0., 0., 0., 0.};
       if (fc & CELL_SOLVED) {
-        double acc[NF] = {0., 0., 0., 0., 0.};
         double AFm[NF], AFp[NF];
         {
           const uint8_t fm = MASK && has_xm ? LF[lane - 1] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
@@ -273,72 +288,56 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
           acc[k] += AFm[k];
           acc[k] -= AFp[k];
         }
-        if (top_wrap_here && m == nm - 1) {
-          double X0[NF], AFw[NF];
-          Tup T0;
-          in.state(g, dt, col, X0);
-          make_tup(X0, T0);
-          const uint8_t f0 = cell_flags<MASK>(g, col);
-          double l;
-          if (MASK && ((f0 ^ fc) & CELL_FLUID)) l = vface(g, true, 3, 1, Tc, true, AFw, ctrl, false);
-          else l = face_flux_area(Tc, T0, 1, P2, A, halfA, AFw);
-          lam_max = (l < lam_max) ? lam_max : l;
-#pragma unroll
-          for (int k = 0; k < NF; ++k)
-            outb[k * fs + idx] = rk_update<STAGE>(g, Uf[k], Tc.c[k], (acc[k] - AFw[k]) + (yact ? AFy[k] : 0.0), dt);
-        } else {
-#pragma unroll
-          for (int k = 0; k < NF; ++k) part[k] = acc[k] + (yact ? AFy[k] : 0.0);
-        }
       }
-      Tp = Tc;
-      fprev = fc;
-      bad_prev = bad_c;
-    }
-    buf ^= 1;
-  }
 
-  if (valid) {
-    const uint32_t pidx = col + (uint32_t)(m1 - 1) * sm;
-    if (top_wrap_here) {
-      if (!(fprev & CELL_SOLVED) && STAGE != 3) {
-#pragma unroll
-        for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
-      }
-    } else {
-      const bool have_next = m1 < nm;
-      double AFy[NF];
-      bool act;
-      double l = 0.0;
-      if (have_next) {
-        double X[NF];
-        Tup Tn;
-        const uint32_t nidx = col + (uint32_t)m1 * sm;
-        in.state(g, dt, nidx, X);
-        make_tup(X, Tn);
-        const uint8_t fn = cell_flags<MASK>(g, nidx);
-        act = (fprev | fn) & CELL_SOLVED;
-        if (act) {
-          if (MASK && ((fprev ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) l = vface(g, true, 2, 1, Tn, false, AFy, ctrl, false);
-            else l = vface(g, true, 3, 1, Tp, true, AFy, ctrl, false);
+      // ---- the y face above this cell (each y face once) ----
+      double AFhi[NF] = {0., 0., 0., 0., 0.};
+      if (has_next) {
+        if ((fc | fn) & CELL_SOLVED) {
+          // the chunk's top face and the periodic wrap face are counted by
+          // the item that holds the row above
+          double l;
+          if (MASK && ((fc ^ fn) & CELL_FLUID)) {
+            if (fn & CELL_FLUID) l = vface(g, true, 2, 1, Tn, false, AFhi, ctrl, in_chunk);
+            else l = vface(g, true, 3, 1, Tc, true, AFhi, ctrl, in_chunk);
           } else {
-            l = face_flux_area(Tp, Tn, 1, P2, A, halfA, AFy);
+            l = face_flux_area(Tc, Tn, 1, P2, A, halfA, AFhi);
+            if (in_chunk && (bad_c | bad_n)) count_neg(Tc, Tn, ctrl);
           }
+          lam_max = (l < lam_max) ? lam_max : l;
         }
-      } else {
-        act = fprev & CELL_SOLVED;
-        if (act) l = vface(g, false, 3, 1, Tp, true, AFy, ctrl, true);
+      } else if (fc & CELL_SOLVED) {  // upper border
+        const double l = vface(g, false, 3, 1, Tc, true, AFhi, ctrl, true);
+        lam_max = (l < lam_max) ? lam_max : l;
       }
-      if (act) lam_max = (l < lam_max) ? lam_max : l;
-      if (fprev & CELL_SOLVED) {
+
+      // ---- this cell is complete: its RK stage update ----
+      if (fc & CELL_SOLVED) {
+        // the periodic top cell: its upper face (the wrap face) comes first
+        const bool wrap_top = top_wrap_here && m == nm - 1;
 #pragma unroll
-        for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = rk_update<STAGE>(g, Uf[k], Tp.c[k], part[k] - AFy[k], dt);
+        for (int k = 0; k < NF; ++k) {
+          const double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
+          outb[k * fs + idx] = rk_update<STAGE>(g, Ucur[k], Tc.c[k], rhs, dt);
+        }
       } else if (STAGE != 3) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) outb[k * fs + pidx] = (STAGE == 1) ? 0.0 : Uf[k];
+        for (int k = 0; k < NF; ++k) outb[k * fs + idx] = (STAGE == 1) ? 0.0 : Ucur[k];
+      }
+#pragma unroll
+      for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
+    }
+    if (build) {
+      Tc = Tn;
+      fc = fn;
+      bad_c = bad_n;
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = Unext[k];
       }
     }
+    __syncthreads();
+    buf ^= 1;
   }
   return lam_max;
 }
