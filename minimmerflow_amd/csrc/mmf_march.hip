@@ -318,10 +318,19 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     }
   }
 
-  // loads of plane m0 + 1 (or of the plane beyond a one-plane chunk)
+  // loads of plane m0 + 1 (or of the plane beyond a one-plane chunk), the
+  // cell flags with them
   Prefetch<STAGE> pf, ph;
-  if (valid && (m0 + 1 < m1 || have_beyond)) pf.issue(src, col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm);
-  if (hslot >= 0 && m0 + 1 < m1) ph.issue(src, hcol + (uint32_t)(m0 + 1) * sm);
+  uint8_t ff = CELL_FLUID | CELL_SOLVED, fh = CELL_FLUID | CELL_SOLVED;
+  if (valid && (m0 + 1 < m1 || have_beyond)) {
+    const uint32_t i1 = col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm;
+    pf.issue(src, i1);
+    ff = cell_flags<MASK>(g, i1);
+  }
+  if (hslot >= 0 && m0 + 1 < m1) {
+    ph.issue(src, hcol + (uint32_t)(m0 + 1) * sm);
+    if (MASK) fh = g.flags[hcol + (uint32_t)(m0 + 1) * sm];
+  }
   __syncthreads();
 
   int buf = 0;
@@ -333,7 +342,6 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     uint8_t *LFn = lfl + (MASK ? (buf ^ 1) * PL : 0);
     const bool in_chunk = m + 1 < m1;                // plane m + 1 is this chunk's
     const bool has_next = in_chunk || have_beyond;  // a cell above this one
-    const uint32_t nplane = in_chunk ? (uint32_t)(m + 1) : beyond;
     if (STAGE == 3 && valid) {
 #pragma unroll
       for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
@@ -355,7 +363,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       }
       make_tup(X, Tn);
       bad_n = tup_negative(Tn);
-      fn = cell_flags<MASK>(g, col + nplane * sm);
+      fn = ff;
       if (in_chunk) {
         tup_store<PL>(Ln, myslot, Tn);
         if (MASK) LFn[myslot] = fn;
@@ -367,12 +375,18 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       ph.state(gp, dt, Xh);
       make_tup(Xh, Th);
       tup_store<PL>(Ln, hslot, Th);
-      if (MASK) LFn[hslot] = g.flags[hcol + (uint32_t)(m + 1) * sm];
+      if (MASK) LFn[hslot] = fh;
     }
     // the loads of plane m + 2 (a whole plane of work to arrive)
-    if (valid && (m + 2 < m1 || (m + 2 == m1 && have_beyond)))
-      pf.issue(src, col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm);
-    if (hslot >= 0 && m + 2 < m1) ph.issue(src, hcol + (uint32_t)(m + 2) * sm);
+    if (valid && (m + 2 < m1 || (m + 2 == m1 && have_beyond))) {
+      const uint32_t i2 = col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm;
+      pf.issue(src, i2);
+      ff = cell_flags<MASK>(g, i2);
+    }
+    if (hslot >= 0 && m + 2 < m1) {
+      ph.issue(src, hcol + (uint32_t)(m + 2) * sm);
+      if (MASK) fh = g.flags[hcol + (uint32_t)(m + 2) * sm];
+    }
 
     if (valid) {
       // ---- in-plane faces of this cell, reference order ----

@@ -202,10 +202,14 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
     }
   }
 
-  // loads of row m0 + 1 (halo lanes only inside the chunk)
+  // loads of row m0 + 1 (halo lanes only inside the chunk), the cell flags with them
   Pf2<STAGE> pf;
-  if ((valid && (m0 + 1 < m1 || have_beyond)) || (halo && m0 + 1 < m1))
-    pf.issue(in, col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm);
+  uint8_t ff = CELL_FLUID | CELL_SOLVED;
+  if ((valid && (m0 + 1 < m1 || have_beyond)) || (halo && m0 + 1 < m1)) {
+    const uint32_t i1 = col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm;
+    pf.issue(in, i1);
+    ff = cell_flags<MASK>(g, i1);
+  }
   __syncthreads();
 
   int buf = 0;
@@ -217,7 +221,6 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
     uint8_t *LFn = lflb + (MASK ? (buf ^ 1) * W2 : 0);
     const bool in_chunk = m + 1 < m1;                // row m + 1 is this chunk's
     const bool has_next = in_chunk || have_beyond;  // a cell above this one
-    const uint32_t nrow = in_chunk ? (uint32_t)(m + 1) : beyond;
     if (STAGE == 3 && valid) {
 #pragma unroll
       for (int k = 0; k < NF; ++k) Ucur[k] = ld(Ub, fs, k, idx);
@@ -239,15 +242,18 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
       }
       make_tup(X, Tn);
       bad_n = tup_negative(Tn);
-      fn = cell_flags<MASK>(g, col + nrow * sm);
+      fn = ff;
       if (in_chunk) {
         tstore(Ln, lane, Tn);
         if (MASK) LFn[lane] = fn;
       }
     }
     // the loads of row m + 2
-    if ((valid && (m + 2 < m1 || (m + 2 == m1 && have_beyond))) || (halo && m + 2 < m1))
-      pf.issue(in, col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm);
+    if ((valid && (m + 2 < m1 || (m + 2 == m1 && have_beyond))) || (halo && m + 2 < m1)) {
+      const uint32_t i2 = col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm;
+      pf.issue(in, i2);
+      ff = cell_flags<MASK>(g, i2);
+    }
 
     if (valid) {
       // ---- x faces of this cell from the LDS neighbours, reference order ----
