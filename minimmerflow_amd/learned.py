@@ -108,16 +108,24 @@ def run_case(c, n, net=None, theta=THETA):
     """Run case c at n^2 to its end time (plain, or learned when net is given).
     Returns (solver, problem, steps, wall seconds of the stepping loop, step
     graphs captured beforehand)."""
-    s, pb = solver(c, n)
-    if net is not None:
-        s.set_learned(net, theta)
-    # capture the step graphs outside the timed loop: with tmax = t the
-    # stage kernels return at once and the state is untouched
-    s.set_time(0.0, 0.0, CFL)
-    s.advance(16)
-    s.advance(1)
-    s.set_time(0.0, end_time(c), CFL)
-    s.synchronize()
+    def make():
+        s, pb = solver(c, n)
+        if net is not None:
+            s.set_learned(net, theta)
+        # capture the step graphs outside the timed loop: with tmax = t the
+        # stage kernels return at once and the state is untouched
+        s.set_time(0.0, 0.0, CFL)
+        s.advance(16)
+        s.advance(1)
+        s.set_time(0.0, end_time(c), CFL)
+        return s, pb
+
+    s, pb = make()
+    # a second instance runs the same case right before the timed one: after
+    # an idle pause the GPU's first ~15 steps of work run up to 16% slower
+    w, _ = make()
+    w.run()
+    w.synchronize()
     t0 = time.perf_counter()
     steps, _ = s.run()
     s.synchronize()
