@@ -715,8 +715,11 @@ hipError_t launch_stage_march2d_o2(const GridDev &g, int chunk, int stage, int m
 }
 
 // ---- persistent batch ---------------------------------------------------------
+// Stage 1 is held to three waves per SIMD (168 registers, 20 B/lane of
+// scratch): 2% faster per step on the 512-case set; stages 2 and 3 need more
+// registers and measured slower at three.
 template <int STAGE>
-__global__ __launch_bounds__(W2) void k_batch2d(const BatchCase *cases, const int *items, int nitems,
+__global__ __launch_bounds__(W2, (STAGE == 1) ? 3 : 2) void k_batch2d(const BatchCase *cases, const int *items, int nitems,
                                                 unsigned int *counter, double *U, double *R, double *W) {
   __shared__ double lds[2 * NTUP * W2];
   __shared__ uint8_t lfl[1];
