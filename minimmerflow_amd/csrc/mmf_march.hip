@@ -120,14 +120,45 @@ struct Prefetch {
   }
 };
 
+// The tuple of each Dirichlet border's virtual state (euler.cpp:261-296: the
+// side's prescribed primitives, whatever the fluid state), built once per
+// block into bct[side * NTUP ...] by the first threads.
+__device__ __forceinline__ void dirichlet_tuples(const GridDev &g, double *bct, int tid) {
+  if (tid < 6 && g.bc[tid] == BC_DIRICHLET) {
+    double v[NF];
+    Tup T;
+    prim2cons(g.info[tid], v);
+    make_tup(v, T);
+    double *o = bct + tid * NTUP;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) o[k] = T.c[k];
+    o[5] = T.vel[0];
+    o[6] = T.vel[1];
+    o[7] = T.vel[2];
+    o[8] = T.p;
+    o[9] = T.a;
+    o[10] = T.H;
+  }
+}
+
 // Area-weighted flux AF of a face whose other side is a virtual state (border
 // side 'side' or a wall), built from the fluid tuple Fl.  fluid_lower: the
 // fluid cell is the lower (L) side of the face.
-__device__ __forceinline__ double virtual_face(const GridDev &g, bool wall, int side, int a, const Tup &Fl,
-                                               bool fluid_lower, double *AF, Ctrl *ctrl, bool count) {
+__device__ __forceinline__ double virtual_face(const GridDev &g, const double *bct, bool wall, int side, int a,
+                                               const Tup &Fl, bool fluid_lower, double *AF, Ctrl *ctrl, bool count) {
   Tup V;
   if (!wall && g.bc[side] == BC_FREE_FLOW) {  // euler.cpp:298-308: a copy
     V = Fl;
+  } else if (!wall && g.bc[side] == BC_DIRICHLET) {
+    const double *o = bct + side * NTUP;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) V.c[k] = o[k];
+    V.vel[0] = o[5];
+    V.vel[1] = o[6];
+    V.vel[2] = o[7];
+    V.p = o[8];
+    V.a = o[9];
+    V.H = o[10];
   } else {
     double v[NF];
     if (wall) reflect_axis(Fl.c, a, v);  // euler.cpp:347-356
@@ -157,10 +188,13 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   static_assert(TY <= 8, "two tuple planes must fit in LDS");
   __shared__ double lds[2 * NTUP * PL];
   __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
+  __shared__ double bct[6 * NTUP];
 
   const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
   const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+  dirichlet_tuples(g, bct, threadIdx.x + TX * threadIdx.y);
+  __syncthreads();
 
   Source<STAGE> src;
   src.X.set(in.X, g.fs);
@@ -292,7 +326,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       }
       if (!have_prev) {  // lower border of the march axis
         if (fc & CELL_SOLVED) {
-          const double l = virtual_face(gp, false, 2 * M, M, Tc, false, AFlo, ctrl, true);
+          const double l = virtual_face(gp, bct, false, 2 * M, M, Tc, false, AFlo, ctrl, true);
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else {
@@ -306,8 +340,8 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         if ((fp | fc) & CELL_SOLVED) {
           double l;
           if (MASK && ((fp ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) l = virtual_face(gp, true, 2 * M, M, Tc, false, AFlo, ctrl, true);
-            else l = virtual_face(gp, true, 2 * M + 1, M, Tp, true, AFlo, ctrl, true);
+            if (fc & CELL_FLUID) l = virtual_face(gp, bct, true, 2 * M, M, Tc, false, AFlo, ctrl, true);
+            else l = virtual_face(gp, bct, true, 2 * M + 1, M, Tp, true, AFlo, ctrl, true);
           } else {
             l = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFlo);
             if (tup_negative(Tp) | bad_c) count_neg(Tp, Tc, ctrl);
@@ -411,7 +445,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               // counted here only when the neighbour does not evaluate this face
               if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
             } else {
-              l = virtual_face(gp, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
+              l = virtual_face(gp, bct, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -425,7 +459,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               l = face_flux_area(Tc, Nb, a, pow2A, A, halfA, AFp);
               if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
             } else {
-              l = virtual_face(gp, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
+              l = virtual_face(gp, bct, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -447,8 +481,8 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           // the block that holds the plane above
           double l;
           if (MASK && ((fc ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) l = virtual_face(gp, true, 2 * M, M, Tn, false, AFhi, ctrl, in_chunk);
-            else l = virtual_face(gp, true, 2 * M + 1, M, Tc, true, AFhi, ctrl, in_chunk);
+            if (fn & CELL_FLUID) l = virtual_face(gp, bct, true, 2 * M, M, Tn, false, AFhi, ctrl, in_chunk);
+            else l = virtual_face(gp, bct, true, 2 * M + 1, M, Tc, true, AFhi, ctrl, in_chunk);
           } else {
             l = face_flux_area(Tc, Tn, M, pow2A, A, halfA, AFhi);
             if (in_chunk && (bad_c | bad_n)) count_neg(Tc, Tn, ctrl);
@@ -456,7 +490,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else if (fc & CELL_SOLVED) {  // upper border of the march axis
-        const double l = virtual_face(gp, false, 2 * M + 1, M, Tc, true, AFhi, ctrl, true);
+        const double l = virtual_face(gp, bct, false, 2 * M + 1, M, Tc, true, AFhi, ctrl, true);
         lam_max = (l < lam_max) ? lam_max : l;
       }
 
