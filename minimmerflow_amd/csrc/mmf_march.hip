@@ -143,7 +143,10 @@ __device__ __forceinline__ void dirichlet_tuples(const GridDev &g, double *bct, 
 
 // Area-weighted flux AF of a face whose other side is a virtual state (border
 // side 'side' or a wall), built from the fluid tuple Fl.  fluid_lower: the
-// fluid cell is the lower (L) side of the face.
+// fluid cell is the lower (L) side of the face.  REFL false: the grid has no
+// body and no reflecting or wall border, so only the copy and Dirichlet forms
+// occur (the reflection, a state conversion per face, is compiled out).
+template <bool REFL>
 __device__ __forceinline__ double virtual_face(const GridDev &g, const double *bct, bool wall, int side, int a,
                                                const Tup &Fl, bool fluid_lower, double *AF, Ctrl *ctrl, bool count) {
   Tup V;
@@ -159,11 +162,13 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, const double *b
     V.p = o[8];
     V.a = o[9];
     V.H = o[10];
-  } else {
+  } else if (REFL) {
     double v[NF];
     if (wall) reflect_axis(Fl.c, a, v);  // euler.cpp:347-356
     else border_state(g, side, a, Fl.c, v);
     make_tup(v, V);
+  } else {
+    V = Fl;  // not reached: no other virtual state in a REFL-free instance
   }
   double lam;
   if (fluid_lower) {
@@ -178,7 +183,7 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, const double *b
 
 }  // namespace
 
-template <int DIM, int TY, int STAGE, bool MASK, bool P2>
+template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
 __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
                                                   Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
   constexpr int NT = TX * TY;
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       }
       if (!have_prev) {  // lower border of the march axis
         if (fc & CELL_SOLVED) {
-          const double l = virtual_face(gp, bct, false, 2 * M, M, Tc, false, AFlo, ctrl, true);
+          const double l = virtual_face<REFL>(gp, bct, false, 2 * M, M, Tc, false, AFlo, ctrl, true);
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else {
@@ -340,8 +345,8 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         if ((fp | fc) & CELL_SOLVED) {
           double l;
           if (MASK && ((fp ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) l = virtual_face(gp, bct, true, 2 * M, M, Tc, false, AFlo, ctrl, true);
-            else l = virtual_face(gp, bct, true, 2 * M + 1, M, Tp, true, AFlo, ctrl, true);
+            if (fc & CELL_FLUID) l = virtual_face<REFL>(gp, bct, true, 2 * M, M, Tc, false, AFlo, ctrl, true);
+            else l = virtual_face<REFL>(gp, bct, true, 2 * M + 1, M, Tp, true, AFlo, ctrl, true);
           } else {
             l = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFlo);
             if (tup_negative(Tp) | bad_c) count_neg(Tp, Tc, ctrl);
@@ -445,7 +450,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               // counted here only when the neighbour does not evaluate this face
               if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
             } else {
-              l = virtual_face(gp, bct, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
+              l = virtual_face<REFL>(gp, bct, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -459,7 +464,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               l = face_flux_area(Tc, Nb, a, pow2A, A, halfA, AFp);
               if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
             } else {
-              l = virtual_face(gp, bct, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
+              l = virtual_face<REFL>(gp, bct, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -481,8 +486,8 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           // the block that holds the plane above
           double l;
           if (MASK && ((fc ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) l = virtual_face(gp, bct, true, 2 * M, M, Tn, false, AFhi, ctrl, in_chunk);
-            else l = virtual_face(gp, bct, true, 2 * M + 1, M, Tc, true, AFhi, ctrl, in_chunk);
+            if (fn & CELL_FLUID) l = virtual_face<REFL>(gp, bct, true, 2 * M, M, Tn, false, AFhi, ctrl, in_chunk);
+            else l = virtual_face<REFL>(gp, bct, true, 2 * M + 1, M, Tc, true, AFhi, ctrl, in_chunk);
           } else {
             l = face_flux_area(Tc, Tn, M, pow2A, A, halfA, AFhi);
             if (in_chunk && (bad_c | bad_n)) count_neg(Tc, Tn, ctrl);
@@ -490,7 +495,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else if (fc & CELL_SOLVED) {  // upper border of the march axis
-        const double l = virtual_face(gp, bct, false, 2 * M + 1, M, Tc, true, AFhi, ctrl, true);
+        const double l = virtual_face<REFL>(gp, bct, false, 2 * M + 1, M, Tc, true, AFhi, ctrl, true);
         lam_max = (l < lam_max) ? lam_max : l;
       }
 
@@ -526,7 +531,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
 }
 
-template <int DIM, int TY, int STAGE, bool MASK, bool P2>
+template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
 static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mode, const double *U, const double *R,
                                  const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
   StageIn<STAGE> in;
@@ -542,19 +547,19 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
   dim3 grid((unsigned)(ntx * nty * nch));
   dim3 block(TX, TY);
-  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK, P2>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx,
+  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK, P2, REFL>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx,
                      nty);
   return hipGetLastError();
 }
 
-template <int DIM, int TY, bool MASK, bool P2>
+template <int DIM, int TY, bool MASK, bool P2, bool REFL>
 static hipError_t launch_march_p(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
   switch (stage) {
-    case 1: return launch_march_t<DIM, TY, 1, MASK, P2>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    case 2: return launch_march_t<DIM, TY, 2, MASK, P2>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    default: return launch_march_t<DIM, TY, 3, MASK, P2>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 1: return launch_march_t<DIM, TY, 1, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return launch_march_t<DIM, TY, 2, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return launch_march_t<DIM, TY, 3, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
 }
 
@@ -562,8 +567,19 @@ template <int DIM, int TY, bool MASK>
 static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
-  if (g.pow2A && g.pow2V) return launch_march_p<DIM, TY, MASK, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-  return launch_march_p<DIM, TY, MASK, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  // a body (walls) or a reflecting / wall border needs the reflected virtual state
+  bool refl = MASK;
+  for (int side = 0; side < 2 * DIM; ++side)
+    refl = refl || ((g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) && !g.periodic[side / 2]);
+  if (refl) {
+    if (g.pow2A && g.pow2V) return launch_march_p<DIM, TY, MASK, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return launch_march_p<DIM, TY, MASK, false, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
+  if constexpr (!MASK) {
+    if (g.pow2A && g.pow2V) return launch_march_p<DIM, TY, false, true, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return launch_march_p<DIM, TY, false, false, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
+  return hipErrorInvalidValue;  // not reached (MASK implies refl)
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {

@@ -48,6 +48,7 @@ CASES = [
     ("3d periodic xyz", 3, (66, 10, 12), [4] * 6, None),
     ("3d periodic x only", 3, (64, 9, 11), [4, 4, 1, 1, 0, 0], None),
     ("3d dirichlet+box", 3, (40, 24, 20), [3, 0, 1, 1, 0, 2], {"boxes": [[-0.4, -0.3, -0.5, 0.1, 0.35, 0.2]]}),
+    ("3d dirichlet free", 3, (40, 24, 20), [3, 0, 0, 3, 0, 3], None),
     ("3d sphere", 3, (36, 36, 36), [3, 0, 0, 0, 0, 0], {"spheres": [[0.1, 0.0, -0.1, 0.35]]}),
     ("2d free", 2, (64, 64), [0] * 4, None),
     ("2d ragged periodic", 2, (100, 37), [4, 4, 4, 4], None),
