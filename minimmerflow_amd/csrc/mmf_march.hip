@@ -148,7 +148,7 @@ __device__ __forceinline__ void dirichlet_tuples(const GridDev &g, double *bct, 
 // occur (the reflection, a state conversion per face, is compiled out).
 template <bool REFL>
 __device__ __forceinline__ double virtual_face(const GridDev &g, const double *bct, bool wall, int side, int a,
-                                               const Tup &Fl, bool fluid_lower, double *AF, Ctrl *ctrl, bool count) {
+                                               const Tup &Fl, bool fluid_lower, double *AF, NegCount &neg, bool count) {
   Tup V;
   if (!wall && g.bc[side] == BC_FREE_FLOW) {  // euler.cpp:298-308: a copy
     V = Fl;
@@ -173,10 +173,10 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, const double *b
   double lam;
   if (fluid_lower) {
     lam = face_flux_area(Fl, V, a, g.pow2A, g.A, g.halfA, AF);
-    if (count) count_neg(Fl, V, ctrl);
+    if (count) neg.add(Fl, V);
   } else {
     lam = face_flux_area(V, Fl, a, g.pow2A, g.A, g.halfA, AF);
-    if (count) count_neg(V, Fl, ctrl);
+    if (count) neg.add(V, Fl);
   }
   return lam;
 }
@@ -288,6 +288,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
+  NegCount neg;  // non-positive states met, summed into the control block at the end
   Tup Tc;  // tuple of this column's cell in plane m
   uint8_t fc = CELL_FLUID | CELL_SOLVED;
   bool bad_c = false;   // Tc has a negative p or rho
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       }
       if (!have_prev) {  // lower border of the march axis
         if (fc & CELL_SOLVED) {
-          const double l = virtual_face<REFL>(gp, bct, false, 2 * M, M, Tc, false, AFlo, ctrl, true);
+          const double l = virtual_face<REFL>(gp, bct, false, 2 * M, M, Tc, false, AFlo, neg, true);
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else {
@@ -345,11 +346,11 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         if ((fp | fc) & CELL_SOLVED) {
           double l;
           if (MASK && ((fp ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) l = virtual_face<REFL>(gp, bct, true, 2 * M, M, Tc, false, AFlo, ctrl, true);
-            else l = virtual_face<REFL>(gp, bct, true, 2 * M + 1, M, Tp, true, AFlo, ctrl, true);
+            if (fc & CELL_FLUID) l = virtual_face<REFL>(gp, bct, true, 2 * M, M, Tc, false, AFlo, neg, true);
+            else l = virtual_face<REFL>(gp, bct, true, 2 * M + 1, M, Tp, true, AFlo, neg, true);
           } else {
             l = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFlo);
-            if (tup_negative(Tp) | bad_c) count_neg(Tp, Tc, ctrl);
+            if (tup_negative(Tp) | bad_c) neg.add(Tp, Tc);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
@@ -448,9 +449,9 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               tup_load<PL>(L, sm_, Nb);
               l = face_flux_area(Nb, Tc, a, pow2A, A, halfA, AFm);
               // counted here only when the neighbour does not evaluate this face
-              if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
+              if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) neg.add(Nb, Tc);
             } else {
-              l = virtual_face<REFL>(gp, bct, wall, 2 * a, a, Tc, false, AFm, ctrl, true);
+              l = virtual_face<REFL>(gp, bct, wall, 2 * a, a, Tc, false, AFm, neg, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -462,9 +463,9 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
               Tup Nb;
               tup_load<PL>(L, sp_, Nb);
               l = face_flux_area(Tc, Nb, a, pow2A, A, halfA, AFp);
-              if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
+              if (bad_c | tup_negative(Nb)) neg.add(Tc, Nb);
             } else {
-              l = virtual_face<REFL>(gp, bct, wall, 2 * a + 1, a, Tc, true, AFp, ctrl, true);
+              l = virtual_face<REFL>(gp, bct, wall, 2 * a + 1, a, Tc, true, AFp, neg, true);
             }
             lam_max = (l < lam_max) ? lam_max : l;
           }
@@ -486,16 +487,16 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
           // the block that holds the plane above
           double l;
           if (MASK && ((fc ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) l = virtual_face<REFL>(gp, bct, true, 2 * M, M, Tn, false, AFhi, ctrl, in_chunk);
-            else l = virtual_face<REFL>(gp, bct, true, 2 * M + 1, M, Tc, true, AFhi, ctrl, in_chunk);
+            if (fn & CELL_FLUID) l = virtual_face<REFL>(gp, bct, true, 2 * M, M, Tn, false, AFhi, neg, in_chunk);
+            else l = virtual_face<REFL>(gp, bct, true, 2 * M + 1, M, Tc, true, AFhi, neg, in_chunk);
           } else {
             l = face_flux_area(Tc, Tn, M, pow2A, A, halfA, AFhi);
-            if (in_chunk && (bad_c | bad_n)) count_neg(Tc, Tn, ctrl);
+            if (in_chunk && (bad_c | bad_n)) neg.add(Tc, Tn);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else if (fc & CELL_SOLVED) {  // upper border of the march axis
-        const double l = virtual_face<REFL>(gp, bct, false, 2 * M + 1, M, Tc, true, AFhi, ctrl, true);
+        const double l = virtual_face<REFL>(gp, bct, false, 2 * M + 1, M, Tc, true, AFhi, neg, true);
         lam_max = (l < lam_max) ? lam_max : l;
       }
 
@@ -528,6 +529,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     __syncthreads();
     buf ^= 1;
   }
+  neg.flush(ctrl, tid);
   block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
 }
 

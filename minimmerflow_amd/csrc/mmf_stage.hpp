@@ -189,6 +189,29 @@ __device__ __forceinline__ void count_neg(const Tup &L, const Tup &R, Ctrl *ctrl
   if (nr) atomicAdd(&ctrl->neg_rho, (unsigned long long)nr);
 }
 
+// Per-thread counts of non-positive p / rho met on face sides (the same
+// totals as count_neg), added to the control block once per wave at the end
+// of a launch instead of with atomics inside the loop.
+struct NegCount {
+  unsigned int p = 0, r = 0;
+  __device__ __forceinline__ void add(const Tup &L, const Tup &R) {
+    p += (unsigned)(L.p < 0.) + (unsigned)(R.p < 0.);
+    r += (unsigned)(L.c[0] < 0.) + (unsigned)(R.c[0] < 0.);
+  }
+  __device__ __forceinline__ void flush(Ctrl *ctrl, int tid) const {
+    unsigned int sp = p, sr = r;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      sp += __shfl_xor(sp, off, 64);
+      sr += __shfl_xor(sr, off, 64);
+    }
+    if ((tid & 63) == 0) {
+      if (sp) atomicAdd(&ctrl->neg_p, (unsigned long long)sp);
+      if (sr) atomicAdd(&ctrl->neg_rho, (unsigned long long)sr);
+    }
+  }
+};
+
 // RK stage update of one field (main.cpp:421,457,493).
 template <int STAGE>
 __device__ __forceinline__ double rk_update(const GridDev &g, double u, double x, double rhs, double dt) {
