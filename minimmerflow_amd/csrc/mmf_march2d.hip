@@ -86,7 +86,7 @@ struct Pf2 {
 };
 
 __device__ __forceinline__ double vface(const GridDev &g, bool wall, int side, int a, const Tup &Fl, bool fluid_lower,
-                                        double *AF, Ctrl *ctrl, bool count) {
+                                        double *AF, NegCount &neg, bool count) {
   Tup V;
   if (!wall && g.bc[side] == BC_FREE_FLOW) {
     V = Fl;
@@ -99,10 +99,10 @@ __device__ __forceinline__ double vface(const GridDev &g, bool wall, int side, i
   double lam;
   if (fluid_lower) {
     lam = face_flux_area(Fl, V, a, g.pow2A, g.A, g.halfA, AF);
-    if (count) count_neg(Fl, V, ctrl);
+    if (count) neg.add(Fl, V);
   } else {
     lam = face_flux_area(V, Fl, a, g.pow2A, g.A, g.halfA, AF);
-    if (count) count_neg(V, Fl, ctrl);
+    if (count) neg.add(V, Fl);
   }
   return lam;
 }
@@ -142,6 +142,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
   const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
+  NegCount neg;  // non-positive states met, added to the item's control block at its end
   Tup Tc;  // tuple of this lane's cell in row m
   uint8_t fc = CELL_FLUID | CELL_SOLVED;
   bool bad_c = false;
@@ -176,7 +177,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
       }
       if (!have_prev) {  // lower border
         if (fc & CELL_SOLVED) {
-          const double l = vface(g, false, 2, 1, Tc, false, AFlo, ctrl, true);
+          const double l = vface(g, false, 2, 1, Tc, false, AFlo, neg, true);
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else {
@@ -190,11 +191,11 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
         if ((fp | fc) & CELL_SOLVED) {
           double l;
           if (MASK && ((fp ^ fc) & CELL_FLUID)) {
-            if (fc & CELL_FLUID) l = vface(g, true, 2, 1, Tc, false, AFlo, ctrl, true);
-            else l = vface(g, true, 3, 1, Tp, true, AFlo, ctrl, true);
+            if (fc & CELL_FLUID) l = vface(g, true, 2, 1, Tc, false, AFlo, neg, true);
+            else l = vface(g, true, 3, 1, Tp, true, AFlo, neg, true);
           } else {
             l = face_flux_area(Tp, Tc, 1, P2, A, halfA, AFlo);
-            if (tup_negative(Tp) | bad_c) count_neg(Tp, Tc, ctrl);
+            if (tup_negative(Tp) | bad_c) neg.add(Tp, Tc);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
@@ -268,9 +269,9 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
             Tup Nb;
             tload(L, lane - 1, Nb);
             l = face_flux_area(Nb, Tc, 0, P2, A, halfA, AFm);
-            if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) count_neg(Nb, Tc, ctrl);
+            if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) neg.add(Nb, Tc);
           } else {
-            l = vface(g, wall, 0, 0, Tc, false, AFm, ctrl, true);
+            l = vface(g, wall, 0, 0, Tc, false, AFm, neg, true);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
@@ -282,9 +283,9 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
             Tup Nb;
             tload(L, lane + 1, Nb);
             l = face_flux_area(Tc, Nb, 0, P2, A, halfA, AFp);
-            if (bad_c | tup_negative(Nb)) count_neg(Tc, Nb, ctrl);
+            if (bad_c | tup_negative(Nb)) neg.add(Tc, Nb);
           } else {
-            l = vface(g, wall, 1, 0, Tc, true, AFp, ctrl, true);
+            l = vface(g, wall, 1, 0, Tc, true, AFp, neg, true);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
@@ -304,16 +305,16 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
           // the item that holds the row above
           double l;
           if (MASK && ((fc ^ fn) & CELL_FLUID)) {
-            if (fn & CELL_FLUID) l = vface(g, true, 2, 1, Tn, false, AFhi, ctrl, in_chunk);
-            else l = vface(g, true, 3, 1, Tc, true, AFhi, ctrl, in_chunk);
+            if (fn & CELL_FLUID) l = vface(g, true, 2, 1, Tn, false, AFhi, neg, in_chunk);
+            else l = vface(g, true, 3, 1, Tc, true, AFhi, neg, in_chunk);
           } else {
             l = face_flux_area(Tc, Tn, 1, P2, A, halfA, AFhi);
-            if (in_chunk && (bad_c | bad_n)) count_neg(Tc, Tn, ctrl);
+            if (in_chunk && (bad_c | bad_n)) neg.add(Tc, Tn);
           }
           lam_max = (l < lam_max) ? lam_max : l;
         }
       } else if (fc & CELL_SOLVED) {  // upper border
-        const double l = vface(g, false, 3, 1, Tc, true, AFhi, ctrl, true);
+        const double l = vface(g, false, 3, 1, Tc, true, AFhi, neg, true);
         lam_max = (l < lam_max) ? lam_max : l;
       }
 
@@ -345,6 +346,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
     __syncthreads();
     buf ^= 1;
   }
+  neg.flush(ctrl, lane);
   return lam_max;
 }
 
