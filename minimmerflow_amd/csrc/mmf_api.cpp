@@ -478,8 +478,9 @@ int enqueue_steps(mmf_ctx *ctx, int64_t nsteps) {
       int rc = launch_stage_timed(ctx, stage, mmf::MODE_STEP);
       if (rc != MMF_OK) return rc;
     }
-    // the tile kernel's stage-3 launch advances the time itself
-    if (kernel_used(ctx) == mmf::KERNEL_TILE) continue;
+    // the tile kernel's and the 3D marching kernel's stage-3 launches advance the time themselves
+    const int used = kernel_used(ctx);
+    if (used == mmf::KERNEL_TILE || (used == mmf::KERNEL_MARCH && ctx->g.dim == 3)) continue;
     hipError_t e = mmf::launch_advance(ctx->dctrl, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "advance kernel");
   }

@@ -531,6 +531,8 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   }
   neg.flush(ctrl, tid);
   block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
+  // stage 3 of a step: the last block advances the time (no separate launch)
+  if (STAGE == 3 && mode == MODE_STEP) close_step(ctrl, tid);
 }
 
 template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
