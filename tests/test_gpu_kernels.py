@@ -81,15 +81,26 @@ def test_march_equals_gather(mmf, orc, name, dim, n, bc, body):
     assert ref[1] == mo
 
 
-def test_march_positivity_counter(mmf, orc):
-    """A negative-pressure state is counted per face side as the oracle counts it."""
-    s, U = build(mmf, 3, [20, 20, 20], [0] * 6)
+# negative densities placed across tile / strip edges: 3D rows 7 | 8 (64 x 8 tiles), 2D columns
+# 61 | 62 (62-cell strips)
+NEG_RHO = {3: [943, 963], 2: [61, 412]}
+
+
+@pytest.mark.parametrize("dim,n", [(3, [20, 20, 20]), (2, [70, 33])])
+def test_march_positivity_counter(mmf, orc, dim, n):
+    """Negative pressures and densities are counted per face side as the oracle
+    counts them (the marching kernels sum per thread and add once per wave)."""
+    s, U = build(mmf, dim, n, [0] * 6)
     U = U.copy()
-    U[[5, 777, 4000], 4] = 0.0  # zero energy -> negative temperature and pressure
-    o = orc.Solver(orc.problem(orc.RADSOD, 3), 1, [20] * 3, [-1.0] * 3, 0.1, side_bc=[0] * 6)
+    U[[5, 777, 2000], 4] = 0.0  # zero energy -> negative temperature and pressure
+    U[NEG_RHO[dim], 0] = -0.5
+    h = 2.0 / max(n)
+    o = orc.Solver(orc.problem(orc.RADSOD, dim), 1, n, [-1.0] * 3, h, side_bc=[0] * 6)
     o.compute_rhs(U)
+    assert o.counters()[0] > 0 and o.counters()[1] > 0
     for kind in (mmf.KERNEL_GATHER, mmf.KERNEL_MARCH):
         s.set_kernel(kind)
+        assert s.kernel()[0] == kind
         s.reset_positivity_counters()
         s.set_state(U)
         s.compute_rhs()

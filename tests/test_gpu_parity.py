@@ -302,15 +302,20 @@ def test_sphere_freestream_steps(mmf, orc):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("key,dim,nc,order", [("vortex_xy", 2, 40, 2), ("radsod", 3, 24, 1), ("radsod", 2, 64, 1)])
-def test_graph_replayed_steps_equal_launched_steps(mmf, key, dim, nc, order):
+@pytest.mark.parametrize("key,dim,nc,order,march", [("vortex_xy", 2, 40, 2, False), ("radsod", 3, 24, 1, False),
+                                                     ("radsod", 2, 64, 1, False), ("radsod", 3, 32, 1, True)])
+def test_graph_replayed_steps_equal_launched_steps(mmf, key, dim, nc, order, march):
     """Steps replayed from captured graphs (16-step and 1-step graphs, with
     host reads, a state upload and a kernel change between replays) equal
-    steps launched one by one."""
+    steps launched one by one.  With march, the 3D marching kernel (whose
+    stage-3 launch advances the time) runs first."""
     pm = mmf.problem(key, dim)
     runs = []
     for graphs in (False, True):
         s = mmf.Solver.for_problem(pm, nc, order=order)
+        if march:
+            s.set_kernel(mmf.KERNEL_MARCH, 8, 8)
+            assert s.kernel()[0] == mmf.KERNEL_MARCH
         s.set_step_graphs(graphs)
         s.set_initial_state(pm)
         s.set_time(0.0, 100.0, 0.45)
