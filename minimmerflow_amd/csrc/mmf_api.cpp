@@ -254,6 +254,7 @@ int mmf_destroy(mmf_ctx *ctx) {
 
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk) {
   if (!ctx) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
   if (kind < mmf::KERNEL_AUTO || kind > mmf::KERNEL_TILE) return fail(ctx, MMF_ERR_INVALID, "unknown kernel kind");
   if (kind == mmf::KERNEL_TILE && !tile_applies(ctx))
     return fail(ctx, MMF_ERR_INVALID, "tile kernel: order 2 in 2D without bodies");
@@ -289,6 +290,7 @@ int mmf_get_sizes(const mmf_ctx *ctx, int64_t *ncells, int64_t *nifaces) {
 
 int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved) {
   if (!ctx) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
   if (ctx->learn_on) return fail(ctx, MMF_ERR_STATE, "the learned reconstruction runs on grids without bodies");
   const int64_t N = ctx->g.N;
   std::vector<uint8_t> f((size_t)N);
@@ -317,6 +319,7 @@ int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved
 
 int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nb, const double *spheres, int32_t ns) {
   if (!ctx || nb < 0 || ns < 0 || (nb && !boxes) || (ns && !spheres)) return fail(ctx, MMF_ERR_INVALID, "bodies");
+  ctx->lam_valid = false;
   if (ctx->learn_on) return fail(ctx, MMF_ERR_STATE, "the learned reconstruction runs on grids without bodies");
   const int64_t N = ctx->g.N;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -408,6 +411,7 @@ int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw) {
 // it holds the 5N doubles of an AoS copy since fs >= N).
 int mmf_set_state(mmf_ctx *ctx, const double *U) {
   if (!ctx || !U) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
   const int64_t N = ctx->g.N, fs = ctx->g.fs;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipMemcpy(ctx->dR, U, sizeof(double) * 5 * (size_t)N, hipMemcpyHostToDevice));
@@ -429,6 +433,7 @@ int mmf_get_state(mmf_ctx *ctx, double *U) {
 
 int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
   if (!ctx) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
   int rc = mmf_ctx_enqueue_rhs(ctx);
   if (rc != MMF_OK) return rc;
   if (rhs) {
@@ -450,6 +455,7 @@ int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
 
 int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl) {
   if (!ctx || !(cfl > 0.0)) return fail(ctx, MMF_ERR_INVALID, "cfl must be positive");
+  ctx->lam_valid = false;
   int rc = read_ctrl(ctx);
   if (rc != MMF_OK) return rc;
   Ctrl c;
@@ -471,6 +477,22 @@ int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl) {
 namespace {
 
 constexpr int GRAPH_STEPS = 16;
+
+// Before steps of the 3D marching kernel with early dt: the maximum of the
+// current state, unless the previous steps left it (outside any capture).
+int ensure_lam(mmf_ctx *ctx) {
+  const bool early = kernel_used(ctx) == mmf::KERNEL_MARCH && mmf::march_early(ctx->g);
+  if (!early) {
+    ctx->lam_valid = false;
+    return MMF_OK;
+  }
+  if (ctx->lam_valid) return MMF_OK;
+  HIPCHK(ctx, hipMemsetAsync(&ctx->dctrl->lam_now, 0, sizeof(unsigned long long), ctx->stream));
+  hipError_t e = mmf::launch_lam_init(ctx->g, ctx->dU, ctx->dctrl, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "early dt");
+  ctx->lam_valid = true;
+  return MMF_OK;
+}
 
 int enqueue_steps(mmf_ctx *ctx, int64_t nsteps) {
   for (int64_t s = 0; s < nsteps; ++s) {
@@ -537,6 +559,8 @@ void mmf_drop_step_graphs(mmf_ctx *ctx) {
 // with kernel timing on they are launched one by one.
 int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
   if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
+  int rc0 = ensure_lam(ctx);
+  if (rc0 != MMF_OK) return rc0;
   if (ctx->timing || !ctx->use_graphs || nsteps == 0) return enqueue_steps(ctx, nsteps);
   std::vector<char> key = launch_key(ctx);
   if (key != ctx->graph_key) {
@@ -643,6 +667,7 @@ int mmf_get_positivity_counters(mmf_ctx *ctx, int64_t *neg_p, int64_t *neg_rho) 
 
 int mmf_reset_positivity_counters(mmf_ctx *ctx) {
   if (!ctx) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
   int rc = read_ctrl(ctx);
   if (rc != MMF_OK) return rc;
   ctx->hctrl->neg_p = 0;

@@ -48,6 +48,10 @@ struct mmf_ctx {
   hipGraphExec_t step_exec[2] = {nullptr, nullptr};
   std::vector<char> graph_key;
   bool use_graphs = true;
+  // Ctrl::lam_now holds the early maximum of the current state (the 3D
+  // marching kernel keeps it up to date from step to step; anything else
+  // that changes the state, the time control or the kernel clears this)
+  bool lam_valid = false;
   double *dFS = nullptr;
   unsigned long long *dlcount = nullptr;
   mmf_output *out = nullptr;

@@ -51,9 +51,15 @@ struct Ctrl {
   double last_eig[3];            // stage maxima of the last completed step
   long long step;
   unsigned long long neg_p, neg_rho;
+  // Early dt (the 3D marching kernel on grids without a body or a reflected
+  // border): the stage-1 maximum of a step, max over cells and axes of
+  // |u_a| + c, known before the step from the state it starts from (bits).
+  // The step's stage-3 launch gathers the next one into lam[k][0].
+  unsigned long long lam_now;
   // running stage maxima (bits) in entry [s][k][0], s = 3: compute_rhs; the
   // stage maximum is the largest over k
   alignas(128) unsigned long long eig[4][EIG_SPLIT][LINE_U64];
+  alignas(128) unsigned long long lam[EIG_SPLIT][LINE_U64];
   // blocks of a step-closing launch that have finished, per line
   // ([k][0]) and lines completed ([EIG_SPLIT][0]); 0 between launches
   unsigned long long done[EIG_SPLIT + 1][LINE_U64];
@@ -96,6 +102,10 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int sta
                               const double *R, const double *W, double *out_R, double *out_W, double *out_U,
                               Ctrl *ctrl, hipStream_t stream);
 bool march_shape_supported(const GridDev &g, const MarchShape &shape);
+// The marching kernel takes dt from Ctrl::lam_now (early dt) on this grid.
+bool march_early(const GridDev &g);
+// Ctrl::lam_now from state U (early dt before a step sequence).
+hipError_t launch_lam_init(const GridDev &g, const double *U, Ctrl *ctrl, hipStream_t stream);
 MarchShape default_march_shape(const GridDev &g, int num_cus);
 hipError_t launch_advance(Ctrl *ctrl, hipStream_t stream);
 // AoS rows (raw order via cell_of_raw, identity when null) <-> SoA planes.

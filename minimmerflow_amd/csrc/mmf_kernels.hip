@@ -193,6 +193,52 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
 // t += dt after stage 3 (main.cpp:505-506); resets the stage maxima.
 __global__ void k_advance(Ctrl *c) { advance_ctrl_wave(c, threadIdx.x); }
 
+// The early maximum of state U: max over cells and axes of |u_a| + c, and
+// over each Dirichlet border's virtual state (euler.cpp:366-376) -- on a grid
+// without a body or a reflected border, what the stage-1 faces of a step
+// starting from U give (each face takes the larger of its two sides, and
+// every cell has faces along every axis).  Ctrl::lam_now is zero beforehand.
+__global__ __launch_bounds__(256) void k_lam_init(GridDev g, const double *U, Ctrl *ctrl) {
+  double m = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += stride) {
+    double c[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) c[k] = U[k * g.fs + i];
+    Tup T;
+    make_tup(c, T);
+    for (int a = 0; a < g.dim; ++a) {
+      const double l = fabs(T.vel[a]) + T.a;
+      m = (l < m) ? m : l;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2 * g.dim) {
+    const int side = threadIdx.x, a = side / 2;
+    if (g.bc[side] == BC_DIRICHLET && !g.periodic[a]) {
+      double v[NF];
+      Tup V;
+      prim2cons(g.info[side], v);
+      make_tup(v, V);
+      const double l = fabs(V.vel[a]) + V.a;
+      m = (l < m) ? m : l;
+    }
+  }
+  __shared__ unsigned long long red[4];
+  unsigned long long b = dbits(m);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off, 64);
+    b = (o > b) ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long r = red[0];
+    for (int w = 1; w < 4; ++w) r = (red[w] > r) ? red[w] : r;
+    if (r) atomicMax(&ctrl->lam_now, r);
+  }
+}
+
 __global__ void k_zero_slot(Ctrl *c, int slot) {
   if (threadIdx.x < EIG_SPLIT) c->eig[slot][threadIdx.x][0] = 0ull;
 }
@@ -237,6 +283,13 @@ hipError_t launch_stage(const GridDev &g, int stage, int mode, const double *U, 
                 : launch_stage_s<3, 1, false>(g, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   return mask ? launch_stage_s<3, 2, true>(g, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
               : launch_stage_s<3, 2, false>(g, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+}
+
+hipError_t launch_lam_init(const GridDev &g, const double *U, Ctrl *ctrl, hipStream_t s) {
+  const int64_t want = (g.N + 255) / 256;
+  const unsigned nb = (unsigned)(want < 2048 ? (want > 0 ? want : 1) : 2048);
+  hipLaunchKernelGGL(k_lam_init, dim3(nb), dim3(256), 0, s, g, U, ctrl);
+  return hipGetLastError();
 }
 
 hipError_t launch_advance(Ctrl *ctrl, hipStream_t s) {

@@ -87,26 +87,28 @@ struct OutFields {
   }
 };
 
-// Stage input of one cell from the stage's source fields.
-template <int STAGE>
+// Stage input of one cell from the stage's source fields.  RB: stage 2
+// rebuilding W1 = U + dt R / V from U and R; otherwise the state X (U in
+// stage 1, W1 in stage 2 with early dt, W in stage 3).
+template <bool RB>
 struct Source {
-  Fields X;  // U (stage 1) or W (stage 3)
-  Fields Uf, Rf;  // stage 2: W1 = U + dt R / V
+  Fields X;
+  Fields Uf, Rf;
   __device__ __forceinline__ void load(const GridDev &g, double dt, uint32_t i, double *c) const {
 #pragma unroll
     for (int k = 0; k < NF; ++k)
-      c[k] = (STAGE == 2) ? Uf.at(k, i) + div_vol(g, dt * Rf.at(k, i)) : X.at(k, i);  // main.cpp:421
+      c[k] = RB ? Uf.at(k, i) + div_vol(g, dt * Rf.at(k, i)) : X.at(k, i);  // main.cpp:421
   }
 };
 
-template <int STAGE>
+template <bool RB>
 struct Prefetch {
-  double a[NF];  // X (stages 1, 3) or U (stage 2)
-  double b[NF];  // R (stage 2)
-  __device__ __forceinline__ void issue(const Source<STAGE> &src, uint32_t i) {
+  double a[NF];  // X, or U (RB)
+  double b[NF];  // R (RB)
+  __device__ __forceinline__ void issue(const Source<RB> &src, uint32_t i) {
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
-      if (STAGE == 2) {
+      if (RB) {
         a[k] = src.Uf.at(k, i);
         b[k] = src.Rf.at(k, i);
       } else {
@@ -116,7 +118,7 @@ struct Prefetch {
   }
   __device__ __forceinline__ void state(const GridDev &g, double dt, double *X) const {
 #pragma unroll
-    for (int k = 0; k < NF; ++k) X[k] = (STAGE == 2) ? a[k] + div_vol(g, dt * b[k]) : a[k];
+    for (int k = 0; k < NF; ++k) X[k] = RB ? a[k] + div_vol(g, dt * b[k]) : a[k];
   }
 };
 
@@ -197,11 +199,19 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 
   const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
-  const double dt = (STAGE >= 2) ? step_dt(ctrl) : 0.0;
+  // Early dt (no body, no reflected border): the step's dt is known before
+  // stage 1 (Ctrl::lam_now), stage 1 writes W1 = U + dt R / V itself, stage 2
+  // reads W1 like stage 3 reads W, and stage 3 gathers the next step's
+  // maximum from the state it writes.  Otherwise stage 1 writes R and stage 2
+  // rebuilds W1 from U and R (RB).
+  constexpr bool EARLY = !MASK && !REFL;
+  constexpr bool RB = STAGE == 2 && !EARLY;
+  const bool w1 = EARLY && STAGE == 1 && mode == MODE_STEP;
+  const double dt = EARLY ? ((mode == MODE_STEP) ? step_dt_early(ctrl) : 0.0) : ((STAGE >= 2) ? step_dt(ctrl) : 0.0);
   dirichlet_tuples(g, bct, threadIdx.x + TX * threadIdx.y);
   __syncthreads();
 
-  Source<STAGE> src;
+  Source<RB> src;
   src.X.set(in.X, g.fs);
   src.Uf.set(in.U, g.fs);
   src.Rf.set(in.R, g.fs);
@@ -288,6 +298,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   const double A = g.A, halfA = g.halfA;
 
   double lam_max = 0.0;
+  double lam_nx = 0.0;  // early dt, stage 3: the next step's maximum over this thread's cells
   NegCount neg;  // non-positive states met, summed into the control block at the end
   Tup Tc;  // tuple of this column's cell in plane m
   uint8_t fc = CELL_FLUID | CELL_SOLVED;
@@ -301,11 +312,11 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   {
     const uint32_t idx0 = col + (uint32_t)m0 * sm;
     if (valid) {
-      Prefetch<STAGE> p0;
+      Prefetch<RB> p0;
       double X[NF];
       p0.issue(src, idx0);
       p0.state(gp, dt, X);
-      if (STAGE == 2) {
+      if (RB) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) Ucur[k] = p0.a[k];
       }
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 
   // loads of plane m0 + 1 (or of the plane beyond a one-plane chunk), the
   // cell flags with them
-  Prefetch<STAGE> pf, ph;
+  Prefetch<RB> pf, ph;
   uint8_t ff = CELL_FLUID | CELL_SOLVED, fh = CELL_FLUID | CELL_SOLVED;
   if (valid && (m0 + 1 < m1 || have_beyond)) {
     const uint32_t i1 = col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm;
@@ -382,7 +393,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     uint8_t *LFn = lfl + (MASK ? (buf ^ 1) * PL : 0);
     const bool in_chunk = m + 1 < m1;                // plane m + 1 is this chunk's
     const bool has_next = in_chunk || have_beyond;  // a cell above this one
-    if (STAGE == 3 && valid) {
+    if (STAGE >= 2 && !RB && valid) {
 #pragma unroll
       for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
     }
@@ -397,7 +408,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     if (valid && has_next) {
       double X[NF];
       pf.state(gp, dt, X);
-      if (STAGE == 2) {
+      if (RB) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) Unext[k] = pf.a[k];
       }
@@ -505,14 +516,26 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
         // the periodic top cell: its upper face (the wrap face) precedes its
         // lower face in the reference's face list
         const bool wrap_top = top_wrap_here && m == nm - 1;
+        double nu[NF];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
           const double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
-          outf.put(k, idx, rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt));
+          nu[k] = w1 ? Tc.c[k] + div_vol(gp, dt * rhs) : rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt);
+          outf.put(k, idx, nu[k]);
+        }
+        if (EARLY && STAGE == 3) {  // the next step's stage-1 eigenvalues of this cell
+          Tup Tq;
+          make_tup(nu, Tq);
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) {
+            const double l = fabs(Tq.vel[a]) + Tq.a;
+            lam_nx = (l < lam_nx) ? lam_nx : l;
+          }
         }
       } else if (STAGE != 3) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) outf.put(k, idx, (STAGE == 1) ? 0.0 : Ucur[k]);
+        for (int k = 0; k < NF; ++k)
+          outf.put(k, idx, (STAGE == 1) ? (w1 ? Tc.c[k] + div_vol(gp, dt * 0.0) : 0.0) : Ucur[k]);
       }
 
       // the next cell's lower face, tuple and U
@@ -521,7 +544,7 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
       Tc = Tn;
       fc = fn;
       bad_c = bad_n;
-      if (STAGE == 2) {
+      if (RB) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) Ucur[k] = Unext[k];
       }
@@ -531,8 +554,19 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   }
   neg.flush(ctrl, tid);
   block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
+  if (EARLY && STAGE == 3 && mode == MODE_STEP) {
+    if (tid == 0) {  // a Dirichlet border's virtual state enters the next maximum too
+#pragma unroll
+      for (int side = 0; side < 2 * DIM; ++side)
+        if (g.bc[side] == BC_DIRICHLET && !g.periodic[side / 2]) {
+          const double l = fabs(bct[side * NTUP + 5 + side / 2]) + bct[side * NTUP + 9];
+          lam_nx = (l < lam_nx) ? lam_nx : l;
+        }
+    }
+    block_lam_to<NT>(ctrl, lam_nx, tid);
+  }
   // stage 3 of a step: the last block advances the time (no separate launch)
-  if (STAGE == 3 && mode == MODE_STEP) close_step(ctrl, tid);
+  if (STAGE == 3 && mode == MODE_STEP) close_step<EARLY>(ctrl, tid);
 }
 
 template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
@@ -543,7 +577,10 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   in.dt = 0.0;
   in.U = U;
   in.R = R;
-  in.X = (STAGE == 3) ? W : U;
+  // with early dt (no body, no reflected border) stage 2 reads the W1 that
+  // stage 1 wrote into R's buffer
+  constexpr bool EARLY = !MASK && !REFL;
+  in.X = (STAGE == 3) ? W : (STAGE == 2 && EARLY) ? R : U;
   double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
   const int M = DIM - 1;
   const int ntx = (int)((g.n[0] + TX - 1) / TX);
@@ -592,6 +629,13 @@ bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
   if (g.order != 1 && !(g.order == 2 && g.dim == 2 && g.flags == nullptr)) return false;
   if (g.dim == 2) return sh.ty == 1;
   return sh.ty == 4 || sh.ty == 8;
+}
+
+bool march_early(const GridDev &g) {
+  if (g.dim != 3 || g.order != 1 || g.flags != nullptr) return false;
+  for (int side = 0; side < 6; ++side)
+    if ((g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) && !g.periodic[side / 2]) return false;
+  return true;
 }
 
 // Longest chunk (fewest boundary planes re-read) that still gives every CU a

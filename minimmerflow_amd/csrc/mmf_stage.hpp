@@ -39,6 +39,20 @@ __device__ __forceinline__ double step_dt(const Ctrl *c) {
   return dt;
 }
 
+// dt of the current step from the early maximum (the same rule).
+__device__ __forceinline__ double step_dt_early(const Ctrl *c) {
+  const double me = __longlong_as_double((long long)c->lam_now);
+  double dt = 0.9 * c->cfl * c->hmin / me;
+  if (c->t + dt > c->tmax) dt = c->tmax - c->t;
+  return dt;
+}
+
+// This block's contribution (bits of a non-negative double) to the next
+// step's early maximum.
+__device__ __forceinline__ void lam_put(Ctrl *c, unsigned long long bits) {
+  if (bits) atomicMax(&c->lam[block_linear() % EIG_SPLIT][0], bits);
+}
+
 // Batched cases keep their stage maxima on line 0 of their control block
 // alone (a case's blocks are few, so one line does not serialise them).
 __device__ __forceinline__ double step_dt_line0(const Ctrl *c) {
@@ -52,17 +66,23 @@ __device__ __forceinline__ double step_dt_line0(const Ctrl *c) {
 // stage maxima kept as the last step's and reset.  Lane 16 s + k reads line
 // k of stage s with an atomic, so a launch whose other blocks formed them may
 // call it.
+// EARLY: dt comes from lam_now, and lanes 48-63 also gather the next step's
+// early maximum from the lam lines into lam_now and reset them.
+template <bool EARLY = false>
 __device__ __forceinline__ void advance_ctrl_wave(Ctrl *c, int lane) {
   unsigned long long v = 0;
   if (lane < 3 * EIG_SPLIT) v = atomicAdd(&c->eig[lane / EIG_SPLIT][lane % EIG_SPLIT][0], 0ull);
+  else if (EARLY) v = atomicAdd(&c->lam[lane - 3 * EIG_SPLIT][0], 0ull);
 #pragma unroll
   for (int off = EIG_SPLIT / 2; off > 0; off >>= 1) {
     const unsigned long long o = __shfl_xor(v, off, 64);
     v = (o > v) ? o : v;
   }
   const unsigned long long e0 = __shfl(v, 0, 64), e1 = __shfl(v, EIG_SPLIT, 64), e2 = __shfl(v, 2 * EIG_SPLIT, 64);
+  const unsigned long long e3 = __shfl(v, 3 * EIG_SPLIT, 64);
   if (lane == 0 && c->t < c->tmax) {
-    double dt = 0.9 * c->cfl * c->hmin / __longlong_as_double((long long)e0);
+    const unsigned long long me = EARLY ? c->lam_now : e0;
+    double dt = 0.9 * c->cfl * c->hmin / __longlong_as_double((long long)me);
     if (c->t + dt > c->tmax) dt = c->tmax - c->t;
     c->dt = dt;
     c->t += dt;
@@ -70,8 +90,10 @@ __device__ __forceinline__ void advance_ctrl_wave(Ctrl *c, int lane) {
     c->last_eig[0] = __longlong_as_double((long long)e0);
     c->last_eig[1] = __longlong_as_double((long long)e1);
     c->last_eig[2] = __longlong_as_double((long long)e2);
+    if (EARLY) c->lam_now = e3;
   }
   if (lane < 3 * EIG_SPLIT) atomicExch(&c->eig[lane / EIG_SPLIT][lane % EIG_SPLIT][0], 0ull);
+  else if (EARLY) atomicExch(&c->lam[lane - 3 * EIG_SPLIT][0], 0ull);
 }
 
 // Called by every thread of every block at the very end of a launch that
@@ -82,6 +104,7 @@ __device__ __forceinline__ void advance_ctrl_wave(Ctrl *c, int lane) {
 // a line counts the line, and the block completing the last line reads every
 // block's maximum with its own atomics.  Only atomics carry the hand-over,
 // so no L2 write-back fence is needed.
+template <bool EARLY = false>
 __device__ __forceinline__ void close_step(Ctrl *ctrl, int tid) {
   __shared__ int last;
   if (tid == 0) {
@@ -97,7 +120,7 @@ __device__ __forceinline__ void close_step(Ctrl *ctrl, int tid) {
   }
   __syncthreads();
   if (last && tid < 64) {
-    advance_ctrl_wave(ctrl, tid);
+    advance_ctrl_wave<EARLY>(ctrl, tid);
     if (tid <= EIG_SPLIT) atomicExch(&ctrl->done[tid][0], 0ull);
   }
 }
@@ -141,6 +164,28 @@ __device__ __forceinline__ void block_max_to(Ctrl *ctrl, int s, double v, int ti
 #pragma unroll
     for (int w = 1; w < (NT + 63) / 64; ++w) m = (red[w] > m) ? red[w] : m;
     stage_max_put(ctrl, s, m);
+  }
+}
+
+// Block max of non-negative doubles (compared as u64 bits) into the next
+// step's early maximum, one atomic per block (on the block's line).
+template <int NT>
+__device__ __forceinline__ void block_lam_to(Ctrl *ctrl, double v, int tid) {
+  __shared__ unsigned long long red[(NT + 63) / 64];
+  unsigned long long b = dbits(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off, 64);
+    b = (o > b) ? o : b;
+  }
+  const int lane = tid & 63, wid = tid >> 6;
+  if (lane == 0) red[wid] = b;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long m = red[0];
+#pragma unroll
+    for (int w = 1; w < (NT + 63) / 64; ++w) m = (red[w] > m) ? red[w] : m;
+    lam_put(ctrl, m);
   }
 }
 
