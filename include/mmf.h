@@ -64,7 +64,7 @@ enum { MMF_CELL_FLUID = 1, MMF_CELL_SOLVED = 2 };
 typedef struct mmf_grid_desc {
     int32_t dim;            /* 2 or 3 */
     int32_t order;          /* 1 (reference) or 2 (limited linear) */
-    int64_t n[3];           /* cells per axis; n[2] ignored in 2D */
+    int64_t n[3];           /* cells per axis, 1 .. 2^30; n[2] ignored in 2D; at most 2^36 cells */
     double origin[3];
     double h;               /* cell edge length */
     int32_t side_bc[6];

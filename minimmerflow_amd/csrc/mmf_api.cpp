@@ -161,8 +161,13 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
   if (d->dim != 2 && d->dim != 3) return MMF_ERR_INVALID;
   if (d->order != 1 && d->order != 2) return MMF_ERR_UNSUPPORTED_ORDER;
   if (!(d->h > 0.0)) return MMF_ERR_INVALID;
+  int64_t ncells = 1;
   for (int a = 0; a < d->dim; ++a) {
-    if (d->n[a] < 1) return MMF_ERR_INVALID;
+    // per-axis counts fit the kernels' 32-bit coordinates; the cell count
+    // bounds the state allocation (5 doubles per cell, three buffers)
+    if (d->n[a] < 1 || d->n[a] > ((int64_t)1 << 30)) return MMF_ERR_INVALID;
+    if (ncells > ((int64_t)1 << 36) / d->n[a]) return MMF_ERR_INVALID;
+    ncells *= d->n[a];
     int lo = d->side_bc[2 * a], hi = d->side_bc[2 * a + 1];
     if (!valid_bc(lo) || !valid_bc(hi)) return MMF_ERR_INVALID;
     if ((lo == MMF_BC_PERIODIC) != (hi == MMF_BC_PERIODIC)) return MMF_ERR_INVALID;

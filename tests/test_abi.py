@@ -57,3 +57,16 @@ def test_batch_create_rejects_bad_cases_without_touching_a_device(mmf):
         with pytest.raises(mmf.MMFError) as e:
             mmf.Batch(descs)
         assert e.value.status == code
+
+
+def test_create_rejects_oversized_grids_without_touching_a_device(mmf):
+    """Grid sizes beyond 2^30 cells per axis or 2^36 cells in all are refused
+    on the host, before the device count is read (which would give
+    ERR_NO_DEVICE here)."""
+    L = mmf.lib()
+    for dim, n in ((2, [1 << 31, 4]), (3, [1 << 13, 1 << 13, 1 << 11]), (3, [1 << 30, 1 << 30, 1 << 30]),
+                   (2, [0, 8])):
+        d = mmf.grid_desc(dim, n, [0.0, 0.0, 0.0], 0.125, [0] * (2 * dim))
+        ctx = ctypes.c_void_p()
+        assert L.mmf_create(ctypes.byref(d), ctypes.byref(ctx)) == mmf.ERR_INVALID, (dim, n)
+        assert not ctx.value
