@@ -329,25 +329,23 @@ int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nb, const double *
   const int64_t N = ctx->g.N;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   uint8_t *dfl = ctx->dflags;
-  if (!dfl) HIPCHK(ctx, hipMalloc(&dfl, (size_t)N));
   double *dbox = nullptr, *dsph = nullptr;
-  if (nb) {
-    HIPCHK(ctx, hipMalloc(&dbox, sizeof(double) * 6 * nb));
-    HIPCHK(ctx, hipMemcpy(dbox, boxes, sizeof(double) * 6 * nb, hipMemcpyHostToDevice));
-  }
-  if (ns) {
-    HIPCHK(ctx, hipMalloc(&dsph, sizeof(double) * 4 * ns));
-    HIPCHK(ctx, hipMemcpy(dsph, spheres, sizeof(double) * 4 * ns, hipMemcpyHostToDevice));
-  }
+  // every temporary is released on every path below
+  hipError_t e = hipSuccess;
+  if (!dfl) e = hipMalloc(&dfl, (size_t)N);
+  if (e == hipSuccess && nb) e = hipMalloc(&dbox, sizeof(double) * 6 * nb);
+  if (e == hipSuccess && nb) e = hipMemcpy(dbox, boxes, sizeof(double) * 6 * nb, hipMemcpyHostToDevice);
+  if (e == hipSuccess && ns) e = hipMalloc(&dsph, sizeof(double) * 4 * ns);
+  if (e == hipSuccess && ns) e = hipMemcpy(dsph, spheres, sizeof(double) * 4 * ns, hipMemcpyHostToDevice);
   GridDev g = ctx->g;
-  hipError_t e = mmf::launch_mark_bodies(g, ctx->desc.origin, dbox, nb, dsph, ns, dfl, ctx->stream);
+  if (e == hipSuccess) e = mmf::launch_mark_bodies(g, ctx->desc.origin, dbox, nb, dsph, ns, dfl, ctx->stream);
   std::vector<uint8_t> h((size_t)N);
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dfl, (size_t)N, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (dbox) (void)hipFree(dbox);
   if (dsph) (void)hipFree(dsph);
   if (e != hipSuccess) {
-    if (dfl != ctx->dflags) (void)hipFree(dfl);
+    if (dfl && dfl != ctx->dflags) (void)hipFree(dfl);
     return hip_fail(ctx, e, "body marking");
   }
   bool all = std::all_of(h.begin(), h.end(), [](uint8_t v) { return v == (MMF_CELL_FLUID | MMF_CELL_SOLVED); });
@@ -801,19 +799,22 @@ int mmf_calibration_copy(int64_t nbytes, int32_t reps, double *ms) {
   if (hipMalloc(&a.p, (size_t)n4 * 32) != hipSuccess) return MMF_ERR_OUT_OF_MEMORY;
   if (hipMalloc(&b.p, (size_t)n4 * 32) != hipSuccess) return MMF_ERR_OUT_OF_MEMORY;
   if (hipMemset(a.p, 0, (size_t)n4 * 32) != hipSuccess) return MMF_ERR_HIP;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return MMF_ERR_HIP;
+  struct Events {  // destroyed on every return path
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  if (hipEventCreate(&ev.e[0]) != hipSuccess || hipEventCreate(&ev.e[1]) != hipSuccess) return MMF_ERR_HIP;
   // warm-up
   if (mmf::launch_copy((const double4 *)a.p, (double4 *)b.p, n4, nullptr) != hipSuccess) return MMF_ERR_HIP;
-  (void)hipEventRecord(e0, nullptr);
+  if (hipEventRecord(ev.e[0], nullptr) != hipSuccess) return MMF_ERR_HIP;
   for (int r = 0; r < reps; ++r)
     if (mmf::launch_copy((const double4 *)a.p, (double4 *)b.p, n4, nullptr) != hipSuccess) return MMF_ERR_HIP;
-  (void)hipEventRecord(e1, nullptr);
-  if (hipEventSynchronize(e1) != hipSuccess) return MMF_ERR_HIP;
+  if (hipEventRecord(ev.e[1], nullptr) != hipSuccess || hipEventSynchronize(ev.e[1]) != hipSuccess) return MMF_ERR_HIP;
   float t = 0.f;
-  (void)hipEventElapsedTime(&t, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (hipEventElapsedTime(&t, ev.e[0], ev.e[1]) != hipSuccess) return MMF_ERR_HIP;
   *ms = (double)t / reps;
   return MMF_OK;
 }
