@@ -5,7 +5,8 @@ The 256^3 radial Sod grid runs the marching kernel at its launch shape
 step's stage 3) and the 192^3 immersed sphere the masked marching kernel with
 the wall reflection; a few SSP-RK3 steps of each are compared bit for bit with
 the oracle's steps (plain C, about 4 s per 256^3 step) from the same initial
-state.  Longer 256^3 runs are checked through properties that hold at any
+state, and so are two steps of each of the 512 cases of the batched
+workload.  Longer 256^3 runs are checked through properties that hold at any
 size: the marching and gather kernels agree bit for bit, total mass is
 conserved while the waves are inside the domain (free-flow borders at rest
 carry no mass flux), and the solution keeps the problem's mirror and
@@ -136,3 +137,36 @@ def test_radsod_256_mass_and_symmetry(mmf):
     exch = np.max(np.abs(F - sw), axis=(0, 1, 2)) / scale
     print("mirror-x deviation per component %s, x<->y %s" % (mir, exch), flush=True)
     assert np.all(mir <= 1e-12) and np.all(exch <= 1e-12)
+
+
+def test_batch_512_mixed_set_steps_bitwise(mmf, orc):
+    """The 512-case mixed set (13.2 M cells: vortices on free-flow grids and
+    radial Sod on reflecting ones, sides 32..256) at the autotuned launch
+    shape: two steps of every case equal the oracle's bit for bit."""
+    from minimmerflow_amd import workloads as WL
+    descs, problems, _ = WL.mixed_2d_cases(mmf, 512, WL.DEFAULT_SEED)
+    b = mmf.Batch(descs)
+    b.set_initial_states(problems)
+    b.set_time(0.0, 1e30, 0.5)
+    chunk, blocks, _ = b.autotune(steps_per_trial=3)
+    print("512 cases, %d cells: %d rows per item, %d blocks" % (b.total_cells, chunk, blocks), flush=True)
+    b.advance(2)
+    t, steps = b.times()
+    keys = {0: orc.VORTEX_XY, 3: orc.RADSOD}
+    same = 0
+    for i, (d, pm) in enumerate(zip(descs, problems)):
+        po = orc.problem(keys[pm.type], 2, vortex_beta=pm.vortex_beta, vortex_center=list(pm.vortex_center),
+                         sod_p_in=pm.sod_p_in, sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out, sod_t_out=pm.sod_t_out)
+        n = [int(d.n[0]), int(d.n[1])]
+        o = orc.Solver(po, 1, n, list(d.origin), d.h, side_bc=[d.side_bc[k] for k in range(6)])
+        U = o.initial_state()
+        W = U.copy()
+        to = 0.0
+        for _ in range(2):
+            dt, _ = o.rk3_step(U, W, 0.5, to, 1e30)
+            to += dt
+        got = b.get_state(i)
+        assert steps[i] == 2 and t[i] == to, (i, n)
+        assert np.array_equal(got, U), (i, n)
+        same += 1
+    print("512 cases, 2 steps: %d bit-identical to the oracle" % same, flush=True)
