@@ -425,7 +425,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only; --gpus %d is not supported "
+        print("bench.py: measures one GPU only; --gpus %d is not supported "
               "(the multi-GPU sharding is a design, not a measured path)" % args.gpus)
         sys.exit(2)
 

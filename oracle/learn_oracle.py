@@ -11,7 +11,7 @@ the face-value evaluation of the limited linear reconstruction
 face density or pressure is not positive), which stays the fallback and is
 restated here exactly as the oracle and the HIP kernels evaluate it.
 
-Definitions shared with the HIP kernels (every choice is this build's own):
+Definitions shared with the HIP kernels (every choice is this package's own, with no reference counterpart):
   * stencil: the 5x5 coarse cells around a cell, conservative components
     rho, rho u, rho v, rho E (VARS); input index v*25 + (dj+2)*5 + (di+2);
   * normalisation per cell and component: c = centre value, s = max(max|S-c|,
