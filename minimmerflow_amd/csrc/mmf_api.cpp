@@ -132,7 +132,13 @@ int read_ctrl(mmf_ctx *ctx) {
 
 }  // namespace
 
+namespace {
+int ensure_tiles(mmf_ctx *ctx);
+}  // namespace
+
 int mmf_ctx_enqueue_rhs(mmf_ctx *ctx) {
+  int rc = ensure_tiles(ctx);
+  if (rc != MMF_OK) return rc;
   hipError_t e = mmf::launch_zero_slot(ctx->dctrl, 3, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "zero slot");
   return launch_stage_timed(ctx, 1, mmf::MODE_RHS_ONLY);
@@ -252,6 +258,7 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->dFS) (void)hipFree(ctx->dFS);
   if (ctx->dlcount) (void)hipFree(ctx->dlcount);
   if (ctx->draw) (void)hipFree(ctx->draw);
+  if (ctx->dtiles) (void)hipFree(ctx->dtiles);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMF_OK;
@@ -260,6 +267,7 @@ int mmf_destroy(mmf_ctx *ctx) {
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk) {
   if (!ctx) return MMF_ERR_INVALID;
   ctx->lam_valid = false;
+  ctx->tiles_valid = false;
   if (kind < mmf::KERNEL_AUTO || kind > mmf::KERNEL_TILE) return fail(ctx, MMF_ERR_INVALID, "unknown kernel kind");
   if (kind == mmf::KERNEL_TILE && !tile_applies(ctx))
     return fail(ctx, MMF_ERR_INVALID, "tile kernel: order 2 in 2D without bodies");
@@ -310,15 +318,18 @@ int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved
     all = all && (v == (MMF_CELL_FLUID | MMF_CELL_SOLVED));
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->tiles_valid = false;
   if (all) {
     if (ctx->dflags) (void)hipFree(ctx->dflags);
     ctx->dflags = nullptr;
     ctx->g.flags = nullptr;
+    ctx->hflags.clear();
     return MMF_OK;
   }
   if (!ctx->dflags) HIPCHK(ctx, hipMalloc(&ctx->dflags, (size_t)N));
   HIPCHK(ctx, hipMemcpy(ctx->dflags, f.data(), (size_t)N, hipMemcpyHostToDevice));
   ctx->g.flags = ctx->dflags;
+  ctx->hflags.swap(f);
   return MMF_OK;
 }
 
@@ -349,13 +360,16 @@ int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nb, const double *
     return hip_fail(ctx, e, "body marking");
   }
   bool all = std::all_of(h.begin(), h.end(), [](uint8_t v) { return v == (MMF_CELL_FLUID | MMF_CELL_SOLVED); });
+  ctx->tiles_valid = false;
   if (all) {
     (void)hipFree(dfl);
     ctx->dflags = nullptr;
     ctx->g.flags = nullptr;
+    ctx->hflags.clear();
   } else {
     ctx->dflags = dfl;
     ctx->g.flags = dfl;
+    ctx->hflags.swap(h);
   }
   return MMF_OK;
 }
@@ -483,6 +497,34 @@ constexpr int GRAPH_STEPS = 16;
 
 // Before steps of the 3D marching kernel with early dt: the maximum of the
 // current state, unless the previous steps left it (outside any capture).
+// The 3D marching kernel's tile table on grids with flags (the masked tiles
+// first; the others run the unmasked code).
+int ensure_tiles(mmf_ctx *ctx) {
+  const bool want = ctx->g.dim == 3 && ctx->g.flags != nullptr && !ctx->hflags.empty() &&
+                    kernel_used(ctx) == mmf::KERNEL_MARCH;
+  if (!want) {
+    ctx->g.march_tiles = nullptr;
+    return MMF_OK;
+  }
+  if (ctx->tiles_valid && ctx->g.march_tiles) return MMF_OK;
+  const int64_t nb = mmf::march_blocks(ctx->g, ctx->shape);
+  std::vector<int32_t> t((size_t)nb);
+  mmf::build_march_tiles(ctx->g, ctx->shape, ctx->hflags.data(), t.data());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->dtiles_n != nb) {
+    if (ctx->dtiles) (void)hipFree(ctx->dtiles);
+    ctx->dtiles = nullptr;
+    ctx->dtiles_n = 0;
+    ctx->g.march_tiles = nullptr;
+    HIPCHK(ctx, hipMalloc(&ctx->dtiles, sizeof(int32_t) * (size_t)nb));
+    ctx->dtiles_n = nb;
+  }
+  HIPCHK(ctx, hipMemcpy(ctx->dtiles, t.data(), sizeof(int32_t) * (size_t)nb, hipMemcpyHostToDevice));
+  ctx->g.march_tiles = ctx->dtiles;
+  ctx->tiles_valid = true;
+  return MMF_OK;
+}
+
 int ensure_lam(mmf_ctx *ctx) {
   const bool early = kernel_used(ctx) == mmf::KERNEL_MARCH && mmf::march_early(ctx->g);
   if (!early) {
@@ -562,6 +604,8 @@ void mmf_drop_step_graphs(mmf_ctx *ctx) {
 // with kernel timing on they are launched one by one.
 int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
   if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
+  int rct = ensure_tiles(ctx);
+  if (rct != MMF_OK) return rct;
   int rc0 = ensure_lam(ctx);
   if (rc0 != MMF_OK) return rc0;
   if (ctx->timing || !ctx->use_graphs || nsteps == 0) return enqueue_steps(ctx, nsteps);

@@ -52,6 +52,12 @@ struct mmf_ctx {
   // marching kernel keeps it up to date from step to step; anything else
   // that changes the state, the time control or the kernel clears this)
   bool lam_valid = false;
+  // cell flags (cell order) kept on the host for the 3D marching kernel's
+  // tile table (empty when every cell is fluid and solved), and the table
+  std::vector<uint8_t> hflags;
+  int32_t *dtiles = nullptr;
+  int64_t dtiles_n = 0;
+  bool tiles_valid = false;
   double *dFS = nullptr;
   unsigned long long *dlcount = nullptr;
   mmf_output *out = nullptr;

@@ -25,7 +25,19 @@ struct GridDev {
   int32_t pow2V;
   int32_t pow2A;       // A is a power of two: A*(0.5*x) == (0.5*A)*x exactly
   double halfA;        // 0.5*A
+  // 3D marching kernel on a grid with flags: per block, its tile, with
+  // MARCH_TILE_MASKED set where the block must read the flags; or nullptr
+  const int32_t *march_tiles;
 };
+
+constexpr int32_t MARCH_TILE_MASKED = 1 << 30;
+
+// Tile of block b of nblk: XCD-aware numbering (consecutive blocks go to the
+// 8 XCDs in turn; each XCD takes a contiguous range of tiles, whose halos
+// then meet in its L2).
+__host__ __device__ inline int march_tile_of_block(int b, int nblk) {
+  return (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
+}
 
 // Stage-kernel variants.
 enum KernelKind { KERNEL_AUTO = 0, KERNEL_GATHER = 1, KERNEL_MARCH = 2, KERNEL_TILE = 3 };
@@ -102,6 +114,10 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int sta
                               const double *R, const double *W, double *out_R, double *out_W, double *out_U,
                               Ctrl *ctrl, hipStream_t stream);
 bool march_shape_supported(const GridDev &g, const MarchShape &shape);
+// Tile table of the 3D marching kernel for cell flags 'flags' (cell order):
+// masked tiles first, the rest in the XCD-aware order.
+void build_march_tiles(const GridDev &g, const MarchShape &shape, const uint8_t *flags, int32_t *tiles);
+int64_t march_blocks(const GridDev &g, const MarchShape &shape);
 // The marching kernel takes dt from Ctrl::lam_now (early dt) on this grid.
 bool march_early(const GridDev &g);
 // Ctrl::lam_now from state U (early dt before a step sequence).

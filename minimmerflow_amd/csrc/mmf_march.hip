@@ -24,6 +24,9 @@
 // 2^29 cells; larger grids use the gather kernel).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "mmf_device.hpp"
 #include "mmf_internal.hpp"
 #include "mmf_stage.hpp"
@@ -185,17 +188,18 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, const double *b
 
 }  // namespace
 
-template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
-__global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
-                                                  Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
+// One block's march over tile 'tile' (x, y tile and chunk).  MASK: the cell
+// flags are read (bodies).  EARLY is the launch's data flow (early dt), so a
+// block of a masked launch that runs the unmasked code keeps it.  lds, lfl
+// and bct are the kernel's LDS arrays.
+template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool EARLY>
+__device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in, const double *U, double *out,
+                                            Ctrl *ctrl, int mode, int chunk, int ntx, int nty, int tile,
+                                            double *lds, uint8_t *lfl, double *bct) {
   constexpr int NT = TX * TY;
   constexpr int M = DIM - 1;  // march axis
   constexpr int RW = TX + 2;
   constexpr int PL = (TY + 2) * RW;
-  static_assert(TY <= 8, "two tuple planes must fit in LDS");
-  __shared__ double lds[2 * NTUP * PL];
-  __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
-  __shared__ double bct[6 * NTUP];
 
   const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
@@ -204,7 +208,6 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   // reads W1 like stage 3 reads W, and stage 3 gathers the next step's
   // maximum from the state it writes.  Otherwise stage 1 writes R and stage 2
   // rebuilds W1 from U and R (RB).
-  constexpr bool EARLY = !MASK && !REFL;
   constexpr bool RB = STAGE == 2 && !EARLY;
   const bool w1 = EARLY && STAGE == 1 && mode == MODE_STEP;
   const double dt = EARLY ? ((mode == MODE_STEP) ? step_dt_early(ctrl) : 0.0) : ((STAGE >= 2) ? step_dt(ctrl) : 0.0);
@@ -220,10 +223,6 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
   OutFields outf;
   outf.set(out, g.fs);
 
-  // XCD-aware tile numbering
-  const int nblk = gridDim.x;
-  const int b = blockIdx.x;
-  const int tile = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
   const int bx = tile % ntx;
   const int by = (tile / ntx) % nty;
   const int bz = tile / (ntx * nty);
@@ -570,6 +569,32 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
 }
 
 template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
+__global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
+                                                  Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
+  constexpr int PL = (TY + 2) * (TX + 2);
+  static_assert(TY <= 8, "two tuple planes must fit in LDS");
+  __shared__ double lds[2 * NTUP * PL];
+  __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
+  __shared__ double bct[6 * NTUP];
+  constexpr bool EARLY = !MASK && !REFL;
+  if (MASK && g.march_tiles != nullptr) {
+    // the tile table: masked tiles first; a tile whose cells, halo ring and
+    // neighbour planes are all fluid and solved runs the unmasked code
+    const int t = g.march_tiles[blockIdx.x];
+    if (t & MARCH_TILE_MASKED)
+      march_block<DIM, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
+                                                         t & ~MARCH_TILE_MASKED, lds, lfl, bct);
+    else
+      march_block<DIM, TY, STAGE, false, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty, t, lds, lfl,
+                                                          bct);
+  } else {
+    march_block<DIM, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
+                                                       march_tile_of_block((int)blockIdx.x, (int)gridDim.x), lds,
+                                                       lfl, bct);
+  }
+}
+
+template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
 static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mode, const double *U, const double *R,
                                  const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
   StageIn<STAGE> in;
@@ -621,6 +646,59 @@ static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int sta
     return launch_march_p<DIM, TY, false, false, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
   return hipErrorInvalidValue;  // not reached (MASK implies refl)
+}
+
+int64_t march_blocks(const GridDev &g, const MarchShape &sh) {
+  const int64_t ntx = (g.n[0] + TX - 1) / TX, nty = (g.n[1] + sh.ty - 1) / sh.ty;
+  return ntx * nty * ((g.n[2] + sh.chunk - 1) / sh.chunk);
+}
+
+void build_march_tiles(const GridDev &g, const MarchShape &sh, const uint8_t *flags, int32_t *tiles) {
+  const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int64_t ntx = (nx + TX - 1) / TX, nty = (ny + sh.ty - 1) / sh.ty;
+  const int nblk = (int)march_blocks(g, sh);
+  const uint8_t all = CELL_FLUID | CELL_SOLVED;
+  // the cells a block reads: its tile and halo ring over its chunk and the
+  // planes below and above it, wrapped across periodic sides
+  auto wrap = [](int64_t i, int64_t n, bool per, int64_t &out) {
+    if (i >= 0 && i < n) { out = i; return true; }
+    if (!per) return false;
+    out = (i + n) % n;
+    return true;
+  };
+  std::vector<uint8_t> masked((size_t)nblk, 0);
+  for (int t = 0; t < nblk; ++t) {
+    const int64_t bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+    const int64_t x0 = bx * TX, y0 = by * sh.ty, m0 = bz * sh.chunk;
+    const int64_t x1 = std::min(x0 + TX, nx), y1 = std::min(y0 + sh.ty, ny), m1 = std::min(m0 + sh.chunk, nz);
+    bool m = false;
+    for (int64_t z = m0 - 1; z <= m1 && !m; ++z) {
+      int64_t zz;
+      if (!wrap(z, nz, g.periodic[2], zz)) continue;
+      for (int64_t y = y0 - 1; y <= y1 && !m; ++y) {
+        int64_t yy;
+        if (!wrap(y, ny, g.periodic[1], yy)) continue;
+        const uint8_t *row = flags + (yy + ny * zz) * nx;
+        for (int64_t x = x0 - 1; x <= x1; ++x) {
+          int64_t xx;
+          if (!wrap(x, nx, g.periodic[0], xx)) continue;
+          if (row[xx] != all) { m = true; break; }
+        }
+      }
+    }
+    masked[(size_t)t] = m;
+  }
+  // masked tiles on the first blocks: they start in the first round and take
+  // longer, so their CUs are not the ones given a second block
+  int k = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int t = march_tile_of_block(b, nblk);
+    if (masked[(size_t)t]) tiles[k++] = t | MARCH_TILE_MASKED;
+  }
+  for (int b = 0; b < nblk; ++b) {
+    const int t = march_tile_of_block(b, nblk);
+    if (!masked[(size_t)t]) tiles[k++] = t;
+  }
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {

@@ -50,6 +50,9 @@ CASES = [
     ("3d dirichlet+box", 3, (40, 24, 20), [3, 0, 1, 1, 0, 2], {"boxes": [[-0.4, -0.3, -0.5, 0.1, 0.35, 0.2]]}),
     ("3d dirichlet free", 3, (40, 24, 20), [3, 0, 0, 3, 0, 3], None),
     ("3d sphere", 3, (36, 36, 36), [3, 0, 0, 0, 0, 0], {"spheres": [[0.1, 0.0, -0.1, 0.35]]}),
+    # a body against the periodic x and y sides: the tiles across the wrap read it
+    ("3d periodic body", 3, (130, 20, 24), [4, 4, 4, 4, 0, 3],
+     {"boxes": [[0.9, -0.75, -0.9, 1.0, -0.68, -0.8]]}),
     ("2d free", 2, (64, 64), [0] * 4, None),
     ("2d ragged periodic", 2, (100, 37), [4, 4, 4, 4], None),
     ("2d box reflecting", 2, (77, 50), [1, 1, 1, 1], {"boxes": [[-0.2, -0.2, -1, 0.3, 0.1, 1]]}),
