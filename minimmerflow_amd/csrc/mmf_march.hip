@@ -688,16 +688,20 @@ void build_march_tiles(const GridDev &g, const MarchShape &sh, const uint8_t *fl
     }
     masked[(size_t)t] = m;
   }
-  // masked tiles on the first blocks: they start in the first round and take
-  // longer, so their CUs are not the ones given a second block
-  int k = 0;
-  for (int b = 0; b < nblk; ++b) {
-    const int t = march_tile_of_block(b, nblk);
-    if (masked[(size_t)t]) tiles[k++] = t | MARCH_TILE_MASKED;
-  }
-  for (int b = 0; b < nblk; ++b) {
-    const int t = march_tile_of_block(b, nblk);
-    if (!masked[(size_t)t]) tiles[k++] = t;
+  // masked tiles on the first blocks of their XCD (the blocks b with the same
+  // b % 8): they start in the first round and take longer, so their CUs are
+  // not the ones given a second block; every tile stays on the XCD of the
+  // XCD-aware numbering, next to its neighbours' halos in that XCD's L2
+  for (int x = 0; x < 8 && x < nblk; ++x) {
+    int k = x;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int b = x; b < nblk; b += 8) {
+        const int t = march_tile_of_block(b, nblk);
+        if ((masked[(size_t)t] != 0) == (pass == 0)) {
+          tiles[k] = pass == 0 ? (t | MARCH_TILE_MASKED) : t;
+          k += 8;
+        }
+      }
   }
 }
 
