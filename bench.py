@@ -349,6 +349,8 @@ def run_learned(args, mmf):
             % (c["kind"], n, P1["l1"], 2 * n, out["plain%d" % (2 * n)]["l1"], 4 * n, out["plain%d" % (4 * n)]["l1"],
                n, L["l1"], L["wall_s"], 2 * n, out["plain%d" % (2 * n)]["wall_s"]))
         cases.append(out)
+        for sv_, _, _, _ in runs.values():
+            sv_.close()
     # stage timing of the learned tile kernel (HIP events)
     c = LR.case_params(1, seed=99)[0]
     sv, _ = LR.solver(c, n)
@@ -359,6 +361,7 @@ def run_learned(args, mmf):
     sv.advance(20)
     stage_ms, launches = sv.read_timing()
     sv.enable_timing(False)
+    sv.close()
     avg = stage_ms / max(1, launches)
     flops = MLP_FLOPS_PER_CELL * n * n
     achieved = flops / (avg * 1e-3) / 1e12

@@ -246,6 +246,9 @@ int mmf_destroy(mmf_ctx *ctx) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
   }
+  // the executable step graphs hold kernel nodes whose arguments point at the
+  // buffers below: release them first, while every pointer is still valid
+  mmf_drop_step_graphs(ctx);
   if (ctx->dU) (void)hipFree(ctx->dU);
   if (ctx->dR) (void)hipFree(ctx->dR);
   if (ctx->dW) (void)hipFree(ctx->dW);
@@ -254,7 +257,6 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->hctrl) (void)hipHostFree(ctx->hctrl);
   if (ctx->dpartials) (void)hipFree(ctx->dpartials);
   if (ctx->derr) (void)hipFree(ctx->derr);
-  mmf_drop_step_graphs(ctx);
   if (ctx->dFS) (void)hipFree(ctx->dFS);
   if (ctx->dlcount) (void)hipFree(ctx->dlcount);
   if (ctx->draw) (void)hipFree(ctx->draw);

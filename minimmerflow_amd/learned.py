@@ -129,7 +129,10 @@ def run_case(c, n, net=None, theta=THETA):
     t0 = time.perf_counter()
     steps, _ = s.run()
     s.synchronize()
-    return s, pb, steps, time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    # released here, not by the garbage collector in the middle of a later case
+    w.close()
+    return s, pb, steps, wall
 
 
 def l1_error(c, s, pb, reference=None, factor=1):

@@ -535,7 +535,11 @@ int orc_solver_init(orc_solver *s, const orc_problem *pb, int order,
         } else {
             int of = s->fluid[m->face_owner[f]];
             int nf = s->fluid[m->face_neigh[f]];
+#ifdef ORC_PLANT_WALL_DROP /* planted mistake (tests only): fluid|solid faces left without a wall */
+            s->face_bc[f] = ORC_BC_NONE + 0 * (of ^ nf);
+#else
             s->face_bc[f] = (of ^ nf) ? ORC_BC_WALL : ORC_BC_NONE;
+#endif
         }
     }
     return 0;
@@ -654,7 +658,17 @@ int orc_compute_rhs(orc_solver *s, const double *U, double *RHS, double *max_eig
         int ownerSolved = s->solved[owner];
         int neighSolved = (neigh >= 0) ? s->solved[neigh] : 0;
 
+#ifndef ORC_PLANT_NO_SKIP /* planted mistake (tests only): faces of unsolved cells evaluated */
         if (!ownerSolved && !neighSolved) continue;
+#else
+        if (neigh >= 0 && s->face_bc[f] == ORC_BC_NONE && !ownerSolved && !neighSolved) {
+            double Fs[ORC_NF], es;
+            orc_eval_splitting(&U[ORC_NF * owner], &U[ORC_NF * neigh], &m->face_normal[3 * f], Fs, &es, NULL, NULL);
+            *max_eig = (es < *max_eig) ? *max_eig : es;
+            continue;
+        }
+        if (!ownerSolved && !neighSolved) continue;
+#endif
 
         int bc = s->face_bc[f];
         const double area = m->area;
@@ -682,6 +696,13 @@ int orc_compute_rhs(orc_solver *s, const double *U, double *RHS, double *max_eig
                 local[1] = -1. * local[1];
                 local[2] = -1. * local[2];
             }
+#ifdef ORC_PLANT_WALL_SLOT /* planted mistake (tests only): the virtual state in the fluid cell's slot */
+            {
+                double *tmp = fluidRec;
+                fluidRec = virtRec;
+                virtRec = tmp;
+            }
+#endif
             orc_reconstruct(s, fluidCell, local, U, fluidRec);
             orc_eval_bc_values(bc, local, &s->face_info[ORC_NF * f], fluidRec, virtRec);
         }
@@ -693,6 +714,10 @@ int orc_compute_rhs(orc_solver *s, const double *U, double *RHS, double *max_eig
 
         *max_eig = (faceMaxEig < *max_eig) ? *max_eig : faceMaxEig; /* std::max(faceMaxEig, *maxEig) */
 
+#ifdef ORC_PLANT_SOLID_UPDATE /* planted mistake (tests only): residuals of unsolved cells accumulated */
+        ownerSolved = 1;
+        neighSolved = neigh >= 0;
+#endif
         if (ownerSolved)
             for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * owner + k] -= area * F[k];
         if (neighSolved)
@@ -722,7 +747,9 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
     if (t + dt > tmax) dt = tmax - t;             /* main.cpp:400-402 */
 
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:409-423 */
+#ifndef ORC_PLANT_SOLID_UPDATE
         if (!s->solved[c]) continue;
+#endif
         for (int k = 0; k < ORC_NF; ++k)
             W[ORC_NF * c + k] = U[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V;
     }
@@ -732,7 +759,9 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
     if (stage_eig) stage_eig[1] = maxEig;
 
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:445-459 */
+#ifndef ORC_PLANT_SOLID_UPDATE
         if (!s->solved[c]) continue;
+#endif
         for (int k = 0; k < ORC_NF; ++k)
 #ifdef ORC_PLANT_RK /* planted mistake (tests only): stage-2 weights of a different scheme */
             W[ORC_NF * c + k] = 0.5 * U[ORC_NF * c + k] + 0.5 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
@@ -746,7 +775,9 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
     if (stage_eig) stage_eig[2] = maxEig;
 
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:481-495 */
+#ifndef ORC_PLANT_SOLID_UPDATE
         if (!s->solved[c]) continue;
+#endif
         for (int k = 0; k < ORC_NF; ++k)
             U[ORC_NF * c + k] = (1. / 3) * U[ORC_NF * c + k] + (2. / 3) * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
     }

@@ -72,6 +72,11 @@ def test_sphere_192_steps_bitwise(mmf, orc):
     gs = mmf.Solver(3, [N_SPHERE] * 3, [-2.0] * 3, h, side_bc, info, 1)
     gs.set_bodies(spheres=[[0.0, 0.0, 0.0, 0.25]])
     fl = gs.cell_flags() & 1
+    # the level set at the centroids (numpy's hypot; no centroid within 1e-12 of the surface)
+    cen = gs.centroids()
+    phi = np.hypot(np.hypot(cen[:, 0], cen[:, 1]), cen[:, 2]) - 0.25
+    assert np.min(np.abs(phi)) > 1e-12
+    assert np.array_equal(fl == 1, phi >= 0.0)
     assert 0 < int((fl == 0).sum()) < N_SPHERE ** 3
     kind, rows, chunk = gs.kernel()
     print("192^3 sphere: kernel %d, tile rows %d, chunk %d, %d solid cells" % (kind, rows, chunk, int((fl == 0).sum())),

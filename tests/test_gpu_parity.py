@@ -147,6 +147,10 @@ def test_rhs_sphere_freestream(mmf, orc):
                            origin=[-2, -2, -2], length=4.0, fs_prim=fs)
     gs.set_bodies(spheres=[[0.0, 0.0, 0.0, 0.5]])
     fl = gs.cell_flags() & 1
+    cen = gs.centroids()
+    phi = np.hypot(np.hypot(cen[:, 0], cen[:, 1]), cen[:, 2]) - 0.5
+    assert np.min(np.abs(phi)) > 1e-12
+    assert np.array_equal(fl == 1, phi >= 0.0)  # the level set at the centroids
     assert 0 < int((fl == 0).sum()) < n ** 3
     os2 = orc.Solver(po, 1, [n] * 3, [-2, -2, -2], 4.0 / n, side_bc=side_bc, side_info=info, fluid=fl)
     U0 = os2.initial_state()
@@ -358,3 +362,27 @@ def test_vortex_64_periodic_order2_to_t10(mmf, orc):
     assert steps == so and t == to
     assert err <= 1e-12
     assert abs(l1g - l1o) <= 1e-12 * l1o
+
+
+def test_error_norms_closed_form(mmf):
+    """mmf_error_norms: a constant density offset delta gives L1 = |delta| |Omega|
+    and L2 = |delta| sqrt(|Omega|) exactly (dyadic data, every sum exact); two
+    offsets on halves give L2 = sqrt((d1^2 + d2^2)/2 |Omega|)."""
+    n, h = 32, 0.25  # |Omega| = 64
+    gs = mmf.Solver(2, [n, n], [0.0, 0.0, 0.0], h, [mmf.BC_FREE_FLOW] * 4)
+    rng = np.random.default_rng(3)
+    U = np.zeros((n * n, 5))
+    U[:, 0] = rng.choice([0.5, 1.0, 2.0], n * n)
+    U[:, 4] = 2.5 * U[:, 0]
+    gs.set_state(U)
+    d = 2.0 ** -6
+    l1, l2 = gs.error_norms(U[:, 0] - d)
+    assert l1 == d * 64.0 and l2 == d * 8.0
+    ex = U[:, 0].copy()
+    ex[: n * n // 2] += 3 * d
+    ex[n * n // 2:] -= 4 * d
+    l1, l2 = gs.error_norms(ex)
+    assert l1 == 3.5 * d * 64.0
+    assert l2 == pytest.approx(np.sqrt(12.5 * d * d * 64.0), rel=1e-15)
+    assert gs.error_norms(U[:, 0]) == (0.0, 0.0)
+    gs.close()
