@@ -133,11 +133,21 @@ int mmf_advance(mmf_ctx *ctx, int64_t nsteps);
  * when the kernel choice, bodies or reconstruction change); on 0 they are
  * launched one by one.  Results are identical either way. */
 int mmf_set_step_graphs(mmf_ctx *ctx, int32_t on);
-/* Step until t >= tmax (or max_steps when >= 0); synchronises. */
+/* Step until t >= tmax (or max_steps when >= 0); synchronises.
+ * Non-finite states: a step whose stage maxima (the face eigenvalues, whose
+ * running maximum keeps a NaN) are NaN or infinite leaves t, dt and the step
+ * count unchanged and is counted; mmf_run then stops and returns
+ * MMF_ERR_STATE (steps_done and t_final filled), as does mmf_get_time.  The
+ * reference has no such check: it logs negative p / rho (euler.cpp:94-101)
+ * and its t would become NaN (main.cpp:398-402, 505). */
 int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_final);
 /* Synchronises and reads the time control block.  stage_eig: the three
- * stage maxima of the last step (main.cpp:389,433,469) or NULL. */
+ * stage maxima of the last step (main.cpp:389,433,469) or NULL.  Returns
+ * MMF_ERR_STATE (outputs filled) once a step met a non-finite state. */
 int mmf_get_time(mmf_ctx *ctx, double *t, double *dt, int64_t *step, double *stage_eig);
+/* steps that met a non-finite state since mmf_set_time or
+ * mmf_reset_positivity_counters (synchronises) */
+int mmf_get_nonfinite_steps(mmf_ctx *ctx, int64_t *steps);
 int mmf_synchronize(mmf_ctx *ctx);
 
 /* ---- output (SolverWriter, solver_writer.cpp:56-119; the writes of
@@ -265,7 +275,9 @@ int mmf_batch_get_shape(const mmf_batch *b, int32_t *chunk, int32_t *blocks);
 int mmf_batch_advance(mmf_batch *b, int64_t nsteps);
 /* steps until every case reaches its end time (max_steps < 0: no limit) */
 int mmf_batch_run(mmf_batch *b, int64_t max_steps, int64_t *steps_done);
-/* per-case time and completed step count (synchronises); either may be NULL */
+/* per-case time and completed step count (synchronises); either may be NULL.
+ * mmf_batch_run and mmf_batch_get_times return MMF_ERR_STATE once a case met
+ * a non-finite state (its time then stays where it was; see mmf_run). */
 int mmf_batch_get_times(mmf_batch *b, double *t, int64_t *steps);
 /* main.cpp:550-561 per case against problems[i] at the case's current time */
 int mmf_batch_error_vs_exact(mmf_batch *b, const mmf_problem *problems, double *l1, double *l2);

@@ -101,6 +101,7 @@ def lib():
         "mmf_set_step_graphs": ([vp, C.c_int32], C.c_int),
         "mmf_run": ([vp, C.c_int64, i64p, dp], C.c_int),
         "mmf_get_time": ([vp, dp, dp, i64p, dp], C.c_int),
+        "mmf_get_nonfinite_steps": ([vp, i64p], C.c_int),
         "mmf_synchronize": ([vp], C.c_int),
         "mmf_write_vtk": ([vp, C.c_char_p], C.c_int),
         "mmf_write_vtk_async": ([vp, C.c_char_p], C.c_int),
@@ -426,6 +427,12 @@ class Solver:
 
     def synchronize(self):
         self._c(lib().mmf_synchronize(self.ctx))
+
+    def nonfinite_steps(self):
+        """Steps that met a NaN or infinite state (their time did not advance)."""
+        n = C.c_int64()
+        self._c(lib().mmf_get_nonfinite_steps(self.ctx, C.byref(n)))
+        return n.value
 
     # output (solver_writer.cpp:56-119)
     def write_vtk(self, path, wait=True):

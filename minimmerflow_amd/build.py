@@ -32,22 +32,51 @@ def hipcc():
     return os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def build(verbose=False):
-    os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + FLAGS + ["-I", INCLUDE, "-I", CSRC] + srcs + ["-o", tmp]
+OBJDIR = os.path.join(HERE, "_build")
+
+
+def _compile(src, verbose):
+    """One translation unit to an object (each keeps its own device code)."""
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    flags = [f for f in FLAGS if f != "-shared"]
+    cmd = [hipcc()] + flags + ["-c", "-I", INCLUDE, "-I", CSRC, src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libmmf.so")
+    return obj, res
+
+
+def _report(res, verbose):
     if verbose and res.stderr.strip():
         # the host pass does not know the packed-fp32 feature; that is harmless
         lines = [l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l]
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source (in parallel, one hipcc per file) and link libmmf.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    jobs = jobs or min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(lambda s: _compile(s, verbose), srcs))
+    for obj, res in results:
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed building %s" % os.path.basename(obj))
+        _report(res, verbose)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + [o for o, _ in results] + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed linking libmmf.so")
+    _report(res, verbose)
     os.replace(tmp, LIB)
     build_driver(verbose)
     return LIB

@@ -130,11 +130,68 @@ int read_ctrl(mmf_ctx *ctx) {
   return MMF_OK;
 }
 
+// MMF_ERR_STATE once a step met a NaN or infinite stage maximum (the control
+// block read last; such steps leave the time unchanged).
+int nonfinite_status(mmf_ctx *ctx) {
+  const Ctrl &c = *ctx->hctrl;
+  if (!c.nonfinite) return MMF_OK;
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "%llu step(s) met a non-finite state (NaN or infinity); t stays at %.17g",
+                (unsigned long long)c.nonfinite, c.t);
+  return fail(ctx, MMF_ERR_STATE, buf);
+}
+
 }  // namespace
 
 namespace {
 int ensure_tiles(mmf_ctx *ctx);
 }  // namespace
+
+namespace mmf {
+
+// The closed-form grid of a descriptor (validated as mmf_create documents);
+// no flags, no tile table, field stride = cells rounded up to 64.
+int grid_from_desc(const mmf_grid_desc *d, GridDev &g) {
+  if (!d) return MMF_ERR_INVALID;
+  if (d->dim != 2 && d->dim != 3) return MMF_ERR_INVALID;
+  if (d->order != 1 && d->order != 2) return MMF_ERR_UNSUPPORTED_ORDER;
+  if (!(d->h > 0.0)) return MMF_ERR_INVALID;
+  int64_t ncells = 1;
+  for (int a = 0; a < d->dim; ++a) {
+    // per-axis counts fit the kernels' 32-bit coordinates; the cell count
+    // bounds the state allocation (5 doubles per cell, three buffers)
+    if (d->n[a] < 1 || d->n[a] > ((int64_t)1 << 30)) return MMF_ERR_INVALID;
+    if (ncells > ((int64_t)1 << 36) / d->n[a]) return MMF_ERR_INVALID;
+    ncells *= d->n[a];
+    int lo = d->side_bc[2 * a], hi = d->side_bc[2 * a + 1];
+    if (!valid_bc(lo) || !valid_bc(hi)) return MMF_ERR_INVALID;
+    if ((lo == MMF_BC_PERIODIC) != (hi == MMF_BC_PERIODIC)) return MMF_ERR_INVALID;
+    if (lo == MMF_BC_PERIODIC && d->n[a] < 2) return MMF_ERR_INVALID;
+  }
+  std::memset(&g, 0, sizeof(g));
+  g.dim = d->dim;
+  g.order = d->order;
+  for (int a = 0; a < 3; ++a) g.n[a] = (a < d->dim) ? d->n[a] : 1;
+  g.N = g.n[0] * g.n[1] * g.n[2];
+  g.fs = (g.N + 63) / 64 * 64;
+  g.h = d->h;
+  g.V = (d->dim == 3) ? d->h * d->h * d->h : d->h * d->h;
+  g.A = (d->dim == 3) ? d->h * d->h : d->h;
+  for (int s = 0; s < 6; ++s) {
+    g.bc[s] = (s / 2 < d->dim) ? d->side_bc[s] : MMF_BC_FREE_FLOW;
+    for (int k = 0; k < 5; ++k) g.info[s][k] = d->side_info[s][k];
+  }
+  for (int a = 0; a < 3; ++a) g.periodic[a] = (a < d->dim) && d->side_bc[2 * a] == MMF_BC_PERIODIC;
+  g.flags = nullptr;
+  int e2 = 0;
+  g.pow2V = std::frexp(g.V, &e2) == 0.5 ? 1 : 0;
+  g.invV = 1.0 / g.V;
+  g.pow2A = std::frexp(g.A, &e2) == 0.5 ? 1 : 0;
+  g.halfA = 0.5 * g.A;
+  return MMF_OK;
+}
+
+}  // namespace mmf
 
 int mmf_ctx_enqueue_rhs(mmf_ctx *ctx) {
   int rc = ensure_tiles(ctx);
@@ -164,21 +221,9 @@ const char *mmf_last_error(const mmf_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
   if (!d || !out) return MMF_ERR_INVALID;
   *out = nullptr;
-  if (d->dim != 2 && d->dim != 3) return MMF_ERR_INVALID;
-  if (d->order != 1 && d->order != 2) return MMF_ERR_UNSUPPORTED_ORDER;
-  if (!(d->h > 0.0)) return MMF_ERR_INVALID;
-  int64_t ncells = 1;
-  for (int a = 0; a < d->dim; ++a) {
-    // per-axis counts fit the kernels' 32-bit coordinates; the cell count
-    // bounds the state allocation (5 doubles per cell, three buffers)
-    if (d->n[a] < 1 || d->n[a] > ((int64_t)1 << 30)) return MMF_ERR_INVALID;
-    if (ncells > ((int64_t)1 << 36) / d->n[a]) return MMF_ERR_INVALID;
-    ncells *= d->n[a];
-    int lo = d->side_bc[2 * a], hi = d->side_bc[2 * a + 1];
-    if (!valid_bc(lo) || !valid_bc(hi)) return MMF_ERR_INVALID;
-    if ((lo == MMF_BC_PERIODIC) != (hi == MMF_BC_PERIODIC)) return MMF_ERR_INVALID;
-    if (lo == MMF_BC_PERIODIC && d->n[a] < 2) return MMF_ERR_INVALID;
-  }
+  GridDev gd;
+  const int rcg = mmf::grid_from_desc(d, gd);
+  if (rcg != MMF_OK) return rcg;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MMF_ERR_NO_DEVICE;
 
@@ -186,28 +231,7 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
   if (!ctx) return MMF_ERR_OUT_OF_MEMORY;
   ctx->desc = *d;
   GridDev &g = ctx->g;
-  std::memset(&g, 0, sizeof(g));
-  g.dim = d->dim;
-  g.order = d->order;
-  for (int a = 0; a < 3; ++a) g.n[a] = (a < d->dim) ? d->n[a] : 1;
-  g.N = g.n[0] * g.n[1] * g.n[2];
-  g.fs = (g.N + 63) / 64 * 64;
-  g.h = d->h;
-  g.V = (d->dim == 3) ? d->h * d->h * d->h : d->h * d->h;
-  g.A = (d->dim == 3) ? d->h * d->h : d->h;
-  for (int s = 0; s < 6; ++s) {
-    g.bc[s] = (s / 2 < d->dim) ? d->side_bc[s] : MMF_BC_FREE_FLOW;
-    for (int k = 0; k < 5; ++k) g.info[s][k] = d->side_info[s][k];
-  }
-  for (int a = 0; a < 3; ++a) g.periodic[a] = (a < d->dim) && d->side_bc[2 * a] == MMF_BC_PERIODIC;
-  g.flags = nullptr;
-  {
-    int e2 = 0;
-    g.pow2V = std::frexp(g.V, &e2) == 0.5 ? 1 : 0;
-    g.invV = 1.0 / g.V;
-    g.pow2A = std::frexp(g.A, &e2) == 0.5 ? 1 : 0;
-    g.halfA = 0.5 * g.A;
-  }
+  g = gd;
   auto cleanup = [&](int code) {
     mmf_destroy(ctx);
     return code;
@@ -647,6 +671,7 @@ int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_fina
   for (;;) {
     const Ctrl &c = *ctx->hctrl;
     int64_t done = c.step - step0;
+    if (c.nonfinite) break;
     if (!(c.t < c.tmax)) break;
     if (max_steps >= 0 && done >= max_steps) break;
     int64_t batch = 1;
@@ -662,7 +687,7 @@ int mmf_run(mmf_ctx *ctx, int64_t max_steps, int64_t *steps_done, double *t_fina
   }
   if (steps_done) *steps_done = ctx->hctrl->step - step0;
   if (t_final) *t_final = ctx->hctrl->t;
-  return MMF_OK;
+  return nonfinite_status(ctx);
 }
 
 int mmf_get_time(mmf_ctx *ctx, double *t, double *dt, int64_t *step, double *stage_eig) {
@@ -675,6 +700,14 @@ int mmf_get_time(mmf_ctx *ctx, double *t, double *dt, int64_t *step, double *sta
   if (step) *step = c.step;
   if (stage_eig)
     for (int s = 0; s < 3; ++s) stage_eig[s] = c.last_eig[s];
+  return nonfinite_status(ctx);
+}
+
+int mmf_get_nonfinite_steps(mmf_ctx *ctx, int64_t *steps) {
+  if (!ctx || !steps) return MMF_ERR_INVALID;
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  *steps = (int64_t)ctx->hctrl->nonfinite;
   return MMF_OK;
 }
 
@@ -721,6 +754,7 @@ int mmf_reset_positivity_counters(mmf_ctx *ctx) {
   if (rc != MMF_OK) return rc;
   ctx->hctrl->neg_p = 0;
   ctx->hctrl->neg_rho = 0;
+  ctx->hctrl->nonfinite = 0;
   HIPCHK(ctx, hipMemcpyAsync(ctx->dctrl, ctx->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return MMF_OK;

@@ -132,6 +132,17 @@ int read_ctrls(mmf_batch *b) {
   return MMF_OK;
 }
 
+// MMF_ERR_STATE once a case's step met a NaN or infinite stage maximum (the
+// control blocks read last; that case's time no longer advances).
+int batch_nonfinite_status(mmf_batch *b) {
+  for (int i = 0; i < b->ncases; ++i)
+    if (b->hctrl[i].nonfinite)
+      return bfail(b, MMF_ERR_STATE, "case " + std::to_string(i) +
+                                         " met a non-finite state (NaN or infinity); its time stays at " +
+                                         std::to_string(b->hctrl[i].t));
+  return MMF_OK;
+}
+
 int collect(mmf_batch *b) {
   for (auto &p : b->events) {
     BCHK(b, hipEventSynchronize(p.second));
@@ -374,6 +385,10 @@ int mmf_batch_run(mmf_batch *b, int64_t max_steps, int64_t *steps_done) {
     bool any = false;
     for (int i = 0; i < b->ncases; ++i) {
       const Ctrl &c = b->hctrl[i];
+      if (c.nonfinite) {
+        if (steps_done) *steps_done = done;
+        return batch_nonfinite_status(b);
+      }
       if (c.t < c.tmax) {
         any = true;
         rem = std::max(rem, c.dt > 0.0 ? std::ceil((c.tmax - c.t) / c.dt) : 1.0);
@@ -398,7 +413,7 @@ int mmf_batch_get_times(mmf_batch *b, double *t, int64_t *steps) {
     if (t) t[i] = b->hctrl[i].t;
     if (steps) steps[i] = b->hctrl[i].step;
   }
-  return MMF_OK;
+  return batch_nonfinite_status(b);
 }
 
 int mmf_batch_error_vs_exact(mmf_batch *b, const mmf_problem *problems, double *l1, double *l2) {

@@ -215,4 +215,14 @@ __device__ __forceinline__ bool tup_negative(const Tup &t) { return (t.p < 0.) |
 // Positive doubles order like their bit patterns: max-reduce through u64.
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
+// Running maximum of face eigenvalues (non-negative doubles, euler.cpp:234's
+// std::max) compared as u64 bits: for finite values the same double as
+// (l < m) ? m : l, and a NaN (whose bits lie above every finite double and
+// +inf) is kept instead of being replaced by the next face's value, so a
+// non-finite state reaches the stage maximum and the step's time control.
+__device__ __forceinline__ double lam_acc(double m, double l) { return (dbits(l) > dbits(m)) ? l : m; }
+
+// Bits at and above those of +inf: the stage maximum is not finite.
+constexpr unsigned long long NONFINITE_BITS = 0x7ff0000000000000ull;
+
 }  // namespace mmf

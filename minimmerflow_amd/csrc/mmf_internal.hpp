@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mmf.h"
+
 namespace mmf {
 
 // Uniform Cartesian grid as the kernels see it (the closed-form
@@ -63,6 +65,10 @@ struct Ctrl {
   double last_eig[3];            // stage maxima of the last completed step
   long long step;
   unsigned long long neg_p, neg_rho;
+  // steps whose stage maxima (or early maximum) were not finite: the state
+  // holds a NaN or infinity; such a step leaves t, dt and the step count
+  // unchanged
+  unsigned long long nonfinite;
   // Early dt (the 3D marching kernel on grids without a body or a reflected
   // border): the stage-1 maximum of a step, max over cells and axes of
   // |u_a| + c, known before the step from the state it starts from (bits).
@@ -106,6 +112,11 @@ inline OutputLayout output_layout(int64_t N) {
   L.total = L.res[4] + 8 * N;
   return L;
 }
+
+// The closed-form grid of a descriptor, validated as mmf_create documents
+// (MMF_OK or the status mmf_create would return); no flags, field stride =
+// cells rounded up to 64.
+int grid_from_desc(const mmf_grid_desc *d, GridDev &g);
 
 // Kernel launchers (mmf_kernels.hip).  All enqueue on 'stream'.
 hipError_t launch_stage(const GridDev &g, int stage, int mode, const double *U, const double *R, const double *W,

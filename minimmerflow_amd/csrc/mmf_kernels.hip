@@ -148,8 +148,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
         double Fm[NF], Fp[NF];
         double lm = face_flux(TLm, TRm, a, Fm);
         double lp = face_flux(TLp, TRp, a, Fp);
-        lam_max = (lm < lam_max) ? lam_max : lm;
-        lam_max = (lp < lam_max) ? lam_max : lp;
+        lam_max = lam_acc(lam_max, lm);
+        lam_max = lam_acc(lam_max, lp);
 
         // positivity counts once per face (the reference evaluates each face once)
         if (mode == MODE_STEP || mode == MODE_RHS_ONLY) {
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_lam_init(GridDev g, const double *U, Ct
     make_tup(c, T);
     for (int a = 0; a < g.dim; ++a) {
       const double l = fabs(T.vel[a]) + T.a;
-      m = (l < m) ? m : l;
+      m = lam_acc(m, l);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x < 2 * g.dim) {
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_lam_init(GridDev g, const double *U, Ct
       prim2cons(g.info[side], v);
       make_tup(v, V);
       const double l = fabs(V.vel[a]) + V.a;
-      m = (l < m) ? m : l;
+      m = lam_acc(m, l);
     }
   }
   __shared__ unsigned long long red[4];

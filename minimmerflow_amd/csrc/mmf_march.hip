@@ -343,7 +343,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
       if (!have_prev) {  // lower border of the march axis
         if (fc & CELL_SOLVED) {
           const double l = virtual_face<REFL>(gp, bct, false, 2 * M, M, Tc, false, AFlo, neg, true);
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
       } else {
         double Xp[NF];
@@ -362,7 +362,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
             l = face_flux_area(Tp, Tc, M, pow2A, A, halfA, AFlo);
             if (tup_negative(Tp) | bad_c) neg.add(Tp, Tc);
           }
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
       }
     }
@@ -463,7 +463,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
             } else {
               l = virtual_face<REFL>(gp, bct, wall, 2 * a, a, Tc, false, AFm, neg, true);
             }
-            lam_max = (l < lam_max) ? lam_max : l;
+            lam_max = lam_acc(lam_max, l);
           }
           {
             const uint8_t fp = MASK && hp ? LF[sp_] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
@@ -477,7 +477,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
             } else {
               l = virtual_face<REFL>(gp, bct, wall, 2 * a + 1, a, Tc, true, AFp, neg, true);
             }
-            lam_max = (l < lam_max) ? lam_max : l;
+            lam_max = lam_acc(lam_max, l);
           }
           if (wrap) {
 #pragma unroll
@@ -503,11 +503,11 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
             l = face_flux_area(Tc, Tn, M, pow2A, A, halfA, AFhi);
             if (in_chunk && (bad_c | bad_n)) neg.add(Tc, Tn);
           }
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
       } else if (fc & CELL_SOLVED) {  // upper border of the march axis
         const double l = virtual_face<REFL>(gp, bct, false, 2 * M + 1, M, Tc, true, AFhi, neg, true);
-        lam_max = (l < lam_max) ? lam_max : l;
+        lam_max = lam_acc(lam_max, l);
       }
 
       // ---- this cell is complete: its RK stage update ----
@@ -528,7 +528,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
 #pragma unroll
           for (int a = 0; a < DIM; ++a) {
             const double l = fabs(Tq.vel[a]) + Tq.a;
-            lam_nx = (l < lam_nx) ? lam_nx : l;
+            lam_nx = lam_acc(lam_nx, l);
           }
         }
       } else if (STAGE != 3) {
@@ -559,7 +559,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
       for (int side = 0; side < 2 * DIM; ++side)
         if (g.bc[side] == BC_DIRICHLET && !g.periodic[side / 2]) {
           const double l = fabs(bct[side * NTUP + 5 + side / 2]) + bct[side * NTUP + 9];
-          lam_nx = (l < lam_nx) ? lam_nx : l;
+          lam_nx = lam_acc(lam_nx, l);
         }
     }
     block_lam_to<NT>(ctrl, lam_nx, tid);

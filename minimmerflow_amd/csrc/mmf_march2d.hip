@@ -178,7 +178,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
       if (!have_prev) {  // lower border
         if (fc & CELL_SOLVED) {
           const double l = vface(g, false, 2, 1, Tc, false, AFlo, neg, true);
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
       } else {
         double Xp[NF];
@@ -197,7 +197,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
             l = face_flux_area(Tp, Tc, 1, P2, A, halfA, AFlo);
             if (tup_negative(Tp) | bad_c) neg.add(Tp, Tc);
           }
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
       }
     }
@@ -273,7 +273,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
           } else {
             l = vface(g, wall, 0, 0, Tc, false, AFm, neg, true);
           }
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
         {
           const uint8_t fp = MASK && has_xp ? LF[lane + 1] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
@@ -287,7 +287,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
           } else {
             l = vface(g, wall, 1, 0, Tc, true, AFp, neg, true);
           }
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
         // first two contributions from zero: their order does not change the bits
 #pragma unroll
@@ -311,11 +311,11 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
             l = face_flux_area(Tc, Tn, 1, P2, A, halfA, AFhi);
             if (in_chunk && (bad_c | bad_n)) neg.add(Tc, Tn);
           }
-          lam_max = (l < lam_max) ? lam_max : l;
+          lam_max = lam_acc(lam_max, l);
         }
       } else if (fc & CELL_SOLVED) {  // upper border
         const double l = vface(g, false, 3, 1, Tc, true, AFhi, neg, true);
-        lam_max = (l < lam_max) ? lam_max : l;
+        lam_max = lam_acc(lam_max, l);
       }
 
       // ---- this cell is complete: its RK stage update ----
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(W2O) void k_march2d_o2(GridDev g, const double *X, 
       } else {  // upper border of the y axis (c == m1 == ny)
         l = border_face2(gg, 3, 1, TyU, true, AFy, ctrl);
       }
-      lam_max = (l < lam_max) ? lam_max : l;
+      lam_max = lam_acc(lam_max, l);
       // row c-1 is complete
       if (c - 1 >= m0) {
         const int r = c - 1;
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(W2O) void k_march2d_o2(GridDev g, const double *X, 
           make_tup(FxU, TU);
           l = border_face2(gg, 1, 0, TU, true, AFx, ctrl);
         }
-        lam_max = (l < lam_max) ? lam_max : l;
+        lam_max = lam_acc(lam_max, l);
       }
 #pragma unroll
       for (int k = 0; k < NF; ++k) saf[k][lane] = AFx[k];
@@ -765,10 +765,15 @@ __global__ void k_batch_advance(const BatchCase *cases, int ncases, unsigned int
   if (i >= ncases) return;
   Ctrl *c = cases[i].ctrl;
   if (c->t < c->tmax) {
-    double dt = step_dt_line0(c);
-    c->dt = dt;
-    c->t += dt;
-    c->step += 1;
+    const unsigned long long e0 = c->eig[0][0][0], e1 = c->eig[1][0][0], e2 = c->eig[2][0][0];
+    if (e0 >= NONFINITE_BITS || e1 >= NONFINITE_BITS || e2 >= NONFINITE_BITS) {
+      c->nonfinite += 1;  // a NaN or infinite state: the time does not advance
+    } else {
+      double dt = step_dt_line0(c);
+      c->dt = dt;
+      c->t += dt;
+      c->step += 1;
+    }
 #pragma unroll
     for (int s = 0; s < 3; ++s) c->last_eig[s] = __longlong_as_double((long long)c->eig[s][0][0]);
   }

@@ -82,15 +82,19 @@ __device__ __forceinline__ void advance_ctrl_wave(Ctrl *c, int lane) {
   const unsigned long long e3 = __shfl(v, 3 * EIG_SPLIT, 64);
   if (lane == 0 && c->t < c->tmax) {
     const unsigned long long me = EARLY ? c->lam_now : e0;
-    double dt = 0.9 * c->cfl * c->hmin / __longlong_as_double((long long)me);
-    if (c->t + dt > c->tmax) dt = c->tmax - c->t;
-    c->dt = dt;
-    c->t += dt;
-    c->step += 1;
     c->last_eig[0] = __longlong_as_double((long long)e0);
     c->last_eig[1] = __longlong_as_double((long long)e1);
     c->last_eig[2] = __longlong_as_double((long long)e2);
     if (EARLY) c->lam_now = e3;
+    if (e0 >= NONFINITE_BITS || e1 >= NONFINITE_BITS || e2 >= NONFINITE_BITS || me >= NONFINITE_BITS) {
+      c->nonfinite += 1;  // a NaN or infinite state: the time does not advance
+    } else {
+      double dt = 0.9 * c->cfl * c->hmin / __longlong_as_double((long long)me);
+      if (c->t + dt > c->tmax) dt = c->tmax - c->t;
+      c->dt = dt;
+      c->t += dt;
+      c->step += 1;
+    }
   }
   if (lane < 3 * EIG_SPLIT) atomicExch(&c->eig[lane / EIG_SPLIT][lane % EIG_SPLIT][0], 0ull);
   else if (EARLY) atomicExch(&c->lam[lane - 3 * EIG_SPLIT][0], 0ull);
