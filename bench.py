@@ -104,34 +104,64 @@ WORKLOADS = {"radsod256": (radsod_setup, 256, "radial_sod_3d"), "sphere192": (sp
 MLP_FLOPS_PER_CELL = 2 * (100 * 64 + 64 * 64 + 64 * 16)
 
 
-def cpu_baseline(n, steps):
-    """The oracle (plain C, one thread) on a bounded sample of the radial-Sod workload."""
-    from oracle import oracle as O
-    pb = O.problem(O.RADSOD, 3, sod_r0=0.4)
-    s = O.Solver(pb, 1, [n] * 3, [-1.0, -1.0, -1.0], 2.0 / n, side_bc=[O.BC_FREE_FLOW] * 6)
-    U = s.initial_state()
-    W = U.copy()
-    t = 0.0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        dt, _ = s.rk3_step(U, W, 0.5, t, 0.25)
-        t += dt
-    el = time.perf_counter() - t0
+def cpu_threads():
+    """Host threads for the all-cores CPU baseline: OMP_NUM_THREADS when set
+    (the share a GPU box grants), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _host():
+    return "%s, %d CPUs visible" % (cpu_model(), os.cpu_count() or 0)
+
+
+def _both(single, multi, threads, what):
+    """cpu_baseline entry: the all-cores rate as the value, the single-thread
+    rate beside it (the reference runs serially per rank)."""
     return {
-        "value": n ** 3 * 3 * steps / el,
+        "value": multi[0],
         "unit": "cell-updates/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": "%d SSP-RK3 step(s) (%d stage updates of %d^3 cells) of the same radial-Sod workload "
-                  "from its initial condition, oracle/mmf_oracle.c -O2, single thread on %s; %.1f s"
-                  % (steps, 3 * steps, n, cpu_model(), el),
+        "sample": "%s; all cores: %s, oracle/mmf_oracle.c -O3 -fopenmp (threaded face loop and RK update, bit for "
+                  "bit the serial oracle) with %d threads on %s" % (what, multi[1], threads, _host()),
+        "single_thread": {"value": single[0], "cores": 1,
+                          "sample": "%s, oracle/mmf_oracle.c -O3, one thread" % single[1]},
     }
 
 
-def cpu_baseline_sphere(gpu_solver, n, budget_s=12.0):
-    """The oracle (plain C, one thread) stepping the immersed-sphere workload
-    (the GPU solver's own fluid flags, Dirichlet inflow) from its initial
-    condition until about budget_s."""
+def cpu_baseline(n, steps, budget_mt=6.0):
+    """The oracle on a bounded sample of the radial-Sod workload: 'steps' RK3
+    steps on one thread, then steps on all host cores for about budget_mt s."""
+    from oracle import oracle as O
+    pb = O.problem(O.RADSOD, 3, sod_r0=0.4)
+    s = O.Solver(pb, 1, [n] * 3, [-1.0, -1.0, -1.0], 2.0 / n, side_bc=[O.BC_FREE_FLOW] * 6)
+    threads = cpu_threads()
+    res = []
+    for nt in (1, threads):
+        U = s.initial_state()
+        W = U.copy()
+        t = 0.0
+        k = 0
+        t0 = time.perf_counter()
+        while (nt == 1 and k < steps) or (nt > 1 and (k < 2 or time.perf_counter() - t0 < budget_mt)):
+            dt, _ = s.rk3_step(U, W, 0.5, t, 0.25, threads=nt)
+            t += dt
+            k += 1
+        el = time.perf_counter() - t0
+        res.append((n ** 3 * 3 * k / el, "%d SSP-RK3 step(s) (%d stage updates of %d^3 cells) in %.1f s" % (k, 3 * k, n, el)))
+    return _both(res[0], res[1], threads, "the same radial-Sod workload from its initial condition")
+
+
+def cpu_baseline_sphere(gpu_solver, n, budget_s=10.0, budget_mt=5.0):
+    """The oracle stepping the immersed-sphere workload (the GPU solver's own
+    fluid flags, Dirichlet inflow) from its initial condition: about budget_s
+    on one thread, then about budget_mt on all host cores."""
     import numpy as np
     from oracle import oracle as O
     fs = [1.0 / GAMMA, 0.3, 0.0, 0.0, 1.0 / GAMMA]
@@ -139,66 +169,78 @@ def cpu_baseline_sphere(gpu_solver, n, budget_s=12.0):
     po = O.problem(O.FFSTEP + 1, 3, fs_prim=fs)  # the uniform free stream
     s = O.Solver(po, 1, [n] * 3, [-2.0] * 3, 4.0 / n, side_bc=[O.BC_DIRICHLET] + [O.BC_FREE_FLOW] * 5,
                  side_info=np.array([fs] * 6), fluid=fluid)
-    U = s.initial_state()
-    W = U.copy()
     solved = int(fluid.sum())
-    t = 0.0
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        dt, _ = s.rk3_step(U, W, 0.5, t, 1e30)
-        t += dt
-        steps += 1
-    el = time.perf_counter() - t0
-    return {
-        "value": solved * 3 * steps / el,
-        "unit": "cell-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": "%d SSP-RK3 step(s) (%d stage updates of the %d solved cells of %d^3) of the same sphere "
-                  "workload from its initial condition, oracle/mmf_oracle.c -O2, single thread on %s; %.1f s"
-                  % (steps, 3 * steps, solved, n, cpu_model(), el),
-    }
+    threads = cpu_threads()
+    res = []
+    for nt, budget in ((1, budget_s), (threads, budget_mt)):
+        U = s.initial_state()
+        W = U.copy()
+        t = 0.0
+        k = 0
+        t0 = time.perf_counter()
+        while k < 1 or time.perf_counter() - t0 < budget:
+            dt, _ = s.rk3_step(U, W, 0.5, t, 1e30, threads=nt)
+            t += dt
+            k += 1
+        el = time.perf_counter() - t0
+        res.append((solved * 3 * k / el, "%d SSP-RK3 step(s) (%d stage updates of the %d solved cells of %d^3) in "
+                    "%.1f s" % (k, 3 * k, solved, n, el)))
+    return _both(res[0], res[1], threads, "the same sphere workload from its initial condition")
 
 
-def cpu_baseline_batch(descs, problems, budget_s=12.0):
-    """The oracle (plain C, one thread) stepping the cases of the batch one
-    after another, one SSP-RK3 step each per pass, from their initial
-    conditions, until about budget_s of stepping."""
+def cpu_baseline_batch(descs, problems, budget_s=10.0, budget_mt=5.0):
+    """The oracle stepping the cases of the batch from their initial
+    conditions, one SSP-RK3 step per case per pass in case order: on one
+    thread for about budget_s, then the cases spread over all host cores
+    (one case per thread at a time) for about budget_mt."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     keys = {0: O.VORTEX_XY, 3: O.RADSOD}
-    cases = []
-    for d, pm in zip(descs, problems):
-        po = O.problem(keys[pm.type], 2, vortex_beta=pm.vortex_beta, vortex_center=list(pm.vortex_center),
-                       sod_p_in=pm.sod_p_in, sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out, sod_t_out=pm.sod_t_out)
-        n = [int(d.n[0]), int(d.n[1])]
-        s = O.Solver(po, 1, n, list(d.origin), d.h, side_bc=[d.side_bc[k] for k in range(6)])
-        U = s.initial_state()
-        cases.append([s, U, U.copy(), 0.0, n[0] * n[1]])
-    cells = 0
-    updates = 0
-    el = 0.0
-    passes = 0
+
+    def make_cases():
+        cases = []
+        for d, pm in zip(descs, problems):
+            po = O.problem(keys[pm.type], 2, vortex_beta=pm.vortex_beta, vortex_center=list(pm.vortex_center),
+                           sod_p_in=pm.sod_p_in, sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out,
+                           sod_t_out=pm.sod_t_out)
+            n = [int(d.n[0]), int(d.n[1])]
+            sv = O.Solver(po, 1, n, list(d.origin), d.h, side_bc=[d.side_bc[k] for k in range(6)])
+            U = sv.initial_state()
+            cases.append([sv, U, U.copy(), 0.0, n[0] * n[1]])
+        return cases
+
+    def step(c):
+        dt, _ = c[0].rk3_step(c[1], c[2], 0.5, c[3], 1e30)
+        c[3] += dt
+        return c[4]
+
+    threads = cpu_threads()
+    res = []
+    cases = make_cases()
+    cells, updates, el, passes = 0, 0, 0.0, 0
     while el < budget_s:
         for c in cases:
             t0 = time.perf_counter()
-            dt, _ = c[0].rk3_step(c[1], c[2], 0.5, c[3], 1e30)
+            cells += step(c)
             el += time.perf_counter() - t0
-            c[3] += dt
             updates += 1
-            cells += c[4]
             if el >= budget_s:
                 break
         passes += 1
-    return {
-        "value": cells * 3 / el,
-        "unit": "cell-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": "%d SSP-RK3 case-steps over the %d cases of the same 512-case set (%d pass(es) in case order "
-                  "from the initial conditions, %d cells stepped), oracle/mmf_oracle.c -O2, single thread on %s; "
-                  "%.1f s" % (updates, len(cases), passes, cells, cpu_model(), el),
-    }
+    res.append((cells * 3 / el, "%d SSP-RK3 case-steps (%d pass(es) in case order, %d cells stepped) in %.1f s"
+                % (updates, passes, cells, el)))
+    cases = make_cases()
+    cells, updates, passes = 0, 0, 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:  # the oracle releases the GIL inside each step
+        while passes < 1 or time.perf_counter() - t0 < budget_mt:
+            cells += sum(ex.map(step, cases))
+            updates += len(cases)
+            passes += 1
+    el = time.perf_counter() - t0
+    res.append((cells * 3 / el, "%d SSP-RK3 case-steps (%d whole pass(es), %d cells stepped) in %.1f s"
+                % (updates, passes, cells, el)))
+    return _both(res[0], res[1], threads, "the %d cases of the same 512-case set" % len(descs))
 
 
 def run_batch(args, mmf):
@@ -286,25 +328,26 @@ def run_batch(args, mmf):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline_order2(n, budget_s=10.0):
-    """The oracle (plain C, one thread): order-2 periodic vortex at n^2, RK3
-    steps from the initial condition until about budget_s."""
+def cpu_baseline_order2(n, budget_s=8.0, budget_mt=4.0):
+    """The oracle: order-2 periodic vortex at n^2, RK3 steps from the initial
+    condition, about budget_s on one thread and budget_mt on all host cores."""
     from oracle import oracle as O
     pb = O.problem(O.VORTEX_XY, 2, vortex_periodic=1)
     s = O.Solver(pb, 2, [n, n], list(pb.domain_origin), pb.domain_length / n, side_bc=[O.BC_PERIODIC] * 4 + [0, 0])
-    U = s.initial_state()
-    W = U.copy()
-    t = 0.0
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        dt, _ = s.rk3_step(U, W, 0.5, t, 100.0)
-        t += dt
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": n * n * 3 * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d SSP-RK3 steps of the plain order-2 periodic vortex at %d^2 (limited linear, Rusanov), "
-                      "oracle/mmf_oracle.c -O2, single thread on %s; %.1f s" % (steps, n, cpu_model(), el)}
+    threads = cpu_threads()
+    res = []
+    for nt, budget in ((1, budget_s), (threads, budget_mt)):
+        U = s.initial_state()
+        W = U.copy()
+        t, k = 0.0, 0
+        t0 = time.perf_counter()
+        while k < 1 or time.perf_counter() - t0 < budget:
+            dt, _ = s.rk3_step(U, W, 0.5, t, 100.0, threads=nt)
+            t += dt
+            k += 1
+        el = time.perf_counter() - t0
+        res.append((n * n * 3 * k / el, "%d SSP-RK3 steps in %.1f s" % (k, el)))
+    return _both(res[0], res[1], threads, "the plain order-2 periodic vortex at %d^2 (limited linear, Rusanov)" % n)
 
 
 def run_learned(args, mmf):

@@ -592,10 +592,11 @@ static int64_t orc_neighbour(const orc_mesh *m, int64_t c, int a, int dir)
  * extension): per cell and axis, minmod-limited undivided differences of the
  * conservative means; a neighbour across a border or inside a body gives a
  * zero one-sided difference. */
-void orc_compute_polynomials(orc_solver *s, const double *U)
+static void orc_polys(orc_solver *s, const double *U, int nthreads)
 {
     if (s->order != 2) return;
     const orc_mesh *m = &s->mesh;
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (int64_t c = 0; c < m->ncells; ++c) {
         for (int a = 0; a < 3; ++a) {
             double *sl = &s->slopes[(3 * c + a) * ORC_NF];
@@ -615,6 +616,8 @@ void orc_compute_polynomials(orc_solver *s, const double *U)
         }
     }
 }
+
+void orc_compute_polynomials(orc_solver *s, const double *U) { orc_polys(s, U, 1); }
 
 /* reconstruction.cpp:67-97.  'outward' is the unit normal of the face
  * pointing out of this cell.  Order 1 copies the mean.  Order 2 evaluates
@@ -643,6 +646,84 @@ void orc_reconstruct(const orc_solver *s, int64_t cell, const double *outward,
         memcpy(values, mean, sizeof(double) * ORC_NF);
 }
 
+/* One interface of euler.cpp:153-247 (the body of the face loop): fluxes of
+ * face f accumulated into the residual of its solved sides, its eigenvalue
+ * into *max_eig, negative p / rho sides counted. */
+static void orc_face(const orc_solver *s, const double *U, double *RHS, int64_t f, double *max_eig,
+                     int64_t *neg_p, int64_t *neg_rho)
+{
+    const orc_mesh *m = &s->mesh;
+    int64_t owner = m->face_owner[f];
+    int64_t neigh = m->face_neigh[f];
+    int ownerSolved = s->solved[owner];
+    int neighSolved = (neigh >= 0) ? s->solved[neigh] : 0;
+
+#ifndef ORC_PLANT_NO_SKIP /* planted mistake (tests only): faces of unsolved cells evaluated */
+    if (!ownerSolved && !neighSolved) return;
+#else
+    if (neigh >= 0 && s->face_bc[f] == ORC_BC_NONE && !ownerSolved && !neighSolved) {
+        double Fs[ORC_NF], es;
+        orc_eval_splitting(&U[ORC_NF * owner], &U[ORC_NF * neigh], &m->face_normal[3 * f], Fs, &es, NULL, NULL);
+        *max_eig = (es < *max_eig) ? *max_eig : es;
+        return;
+    }
+    if (!ownerSolved && !neighSolved) return;
+#endif
+
+    int bc = s->face_bc[f];
+    const double area = m->area;
+    const double *normal = &m->face_normal[3 * f];
+
+    double ownerRec[ORC_NF], neighRec[ORC_NF];
+    if (bc == ORC_BC_NONE) {
+        double flipped[3] = {-normal[0], -normal[1], -normal[2]};
+        orc_reconstruct(s, owner, normal, U, ownerRec);
+        orc_reconstruct(s, neigh, flipped, U, neighRec);
+    } else {
+        /* euler.cpp:198-225: the fluid side's normal points out of it */
+        double local[3] = {normal[0], normal[1], normal[2]};
+        int64_t fluidCell;
+        double *fluidRec, *virtRec;
+        if (ownerSolved) {
+            fluidCell = owner;
+            fluidRec = ownerRec;
+            virtRec = neighRec;
+        } else {
+            fluidCell = neigh;
+            fluidRec = neighRec;
+            virtRec = ownerRec;
+            local[0] = -1. * local[0];
+            local[1] = -1. * local[1];
+            local[2] = -1. * local[2];
+        }
+#ifdef ORC_PLANT_WALL_SLOT /* planted mistake (tests only): the virtual state in the fluid cell's slot */
+        {
+            double *tmp = fluidRec;
+            fluidRec = virtRec;
+            virtRec = tmp;
+        }
+#endif
+        orc_reconstruct(s, fluidCell, local, U, fluidRec);
+        orc_eval_bc_values(bc, local, &s->face_info[ORC_NF * f], fluidRec, virtRec);
+    }
+
+    double F[ORC_NF];
+    double faceMaxEig;
+    /* euler.cpp:232: the flux uses the owner-outward normal */
+    orc_eval_splitting(ownerRec, neighRec, normal, F, &faceMaxEig, neg_p, neg_rho);
+
+    *max_eig = (faceMaxEig < *max_eig) ? *max_eig : faceMaxEig; /* std::max(faceMaxEig, *maxEig) */
+
+#ifdef ORC_PLANT_SOLID_UPDATE /* planted mistake (tests only): residuals of unsolved cells accumulated */
+    ownerSolved = 1;
+    neighSolved = neigh >= 0;
+#endif
+    if (ownerSolved)
+        for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * owner + k] -= area * F[k];
+    if (neighSolved)
+        for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * neigh + k] += area * F[k];
+}
+
 /* euler.cpp:127-249 */
 int orc_compute_rhs(orc_solver *s, const double *U, double *RHS, double *max_eig)
 {
@@ -652,91 +733,84 @@ int orc_compute_rhs(orc_solver *s, const double *U, double *RHS, double *max_eig
     for (int64_t i = 0; i < m->ncells * ORC_NF; ++i) RHS[i] = 0.;
 
     *max_eig = 0.0;
-    for (int64_t f = 0; f < m->nfaces; ++f) {
-        int64_t owner = m->face_owner[f];
-        int64_t neigh = m->face_neigh[f];
-        int ownerSolved = s->solved[owner];
-        int neighSolved = (neigh >= 0) ? s->solved[neigh] : 0;
+    for (int64_t f = 0; f < m->nfaces; ++f)
+        orc_face(s, U, RHS, f, max_eig, &s->neg_p, &s->neg_rho);
+    return 0;
+}
 
-#ifndef ORC_PLANT_NO_SKIP /* planted mistake (tests only): faces of unsolved cells evaluated */
-        if (!ownerSolved && !neighSolved) continue;
-#else
-        if (neigh >= 0 && s->face_bc[f] == ORC_BC_NONE && !ownerSolved && !neighSolved) {
-            double Fs[ORC_NF], es;
-            orc_eval_splitting(&U[ORC_NF * owner], &U[ORC_NF * neigh], &m->face_normal[3 * f], Fs, &es, NULL, NULL);
-            *max_eig = (es < *max_eig) ? *max_eig : es;
-            continue;
-        }
-        if (!ownerSolved && !neighSolved) continue;
-#endif
-
-        int bc = s->face_bc[f];
-        const double area = m->area;
-        const double *normal = &m->face_normal[3 * f];
-
-        double ownerRec[ORC_NF], neighRec[ORC_NF];
-        if (bc == ORC_BC_NONE) {
-            double flipped[3] = {-normal[0], -normal[1], -normal[2]};
-            orc_reconstruct(s, owner, normal, U, ownerRec);
-            orc_reconstruct(s, neigh, flipped, U, neighRec);
-        } else {
-            /* euler.cpp:198-225: the fluid side's normal points out of it */
-            double local[3] = {normal[0], normal[1], normal[2]};
-            int64_t fluidCell;
-            double *fluidRec, *virtRec;
-            if (ownerSolved) {
-                fluidCell = owner;
-                fluidRec = ownerRec;
-                virtRec = neighRec;
+/* The same residual with the face loop spread over threads, bit for bit the
+ * serial one: the faces of one axis are split into groups whose cells are
+ * disjoint (x: rows; y: (plane, 64-column chunk); z: (row, 64-column chunk)),
+ * each group runs its faces in increasing position along the axis, and the
+ * axes run one after another, so every cell adds its faces in the serial
+ * order (all x, then y, then z; lower before upper).  The eigenvalue maximum
+ * and the counters are combined per thread (a maximum of finite values does
+ * not depend on the order). */
+int orc_compute_rhs_threads(orc_solver *s, const double *U, double *RHS, double *max_eig, int nthreads)
+{
+    const orc_mesh *m = &s->mesh;
+    const int64_t n0 = m->n[0], n1 = m->n[1], n2 = m->n[2];
+    const int64_t CH = 64;
+    const int64_t nch = (n0 + CH - 1) / CH;
+    double gmax = 0.0;
+    int64_t gp = 0, gr = 0;
+#pragma omp parallel num_threads(nthreads) if (nthreads > 1)
+    {
+        double lmax = 0.0;
+        int64_t lp = 0, lr = 0;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < m->ncells * ORC_NF; ++i) RHS[i] = 0.;
+        int64_t base = 0;
+        for (int a = 0; a < m->dim; ++a) {
+            const int64_t lim = m->periodic[a] ? m->n[a] : m->n[a] + 1;
+            if (a == 0) {
+#pragma omp for schedule(static)
+                for (int64_t r = 0; r < n1 * n2; ++r)
+                    for (int64_t p = 0; p < lim; ++p) orc_face(s, U, RHS, base + p + lim * r, &lmax, &lp, &lr);
+            } else if (a == 1) {
+#pragma omp for schedule(static)
+                for (int64_t t = 0; t < n2 * nch; ++t) {
+                    const int64_t k = t / nch, i0 = (t % nch) * CH, i1 = (i0 + CH < n0) ? i0 + CH : n0;
+                    for (int64_t p = 0; p < lim; ++p)
+                        for (int64_t i = i0; i < i1; ++i)
+                            orc_face(s, U, RHS, base + i + n0 * (p + lim * k), &lmax, &lp, &lr);
+                }
             } else {
-                fluidCell = neigh;
-                fluidRec = neighRec;
-                virtRec = ownerRec;
-                local[0] = -1. * local[0];
-                local[1] = -1. * local[1];
-                local[2] = -1. * local[2];
+#pragma omp for schedule(static)
+                for (int64_t t = 0; t < n1 * nch; ++t) {
+                    const int64_t j = t / nch, i0 = (t % nch) * CH, i1 = (i0 + CH < n0) ? i0 + CH : n0;
+                    for (int64_t p = 0; p < lim; ++p)
+                        for (int64_t i = i0; i < i1; ++i)
+                            orc_face(s, U, RHS, base + i + n0 * (j + n1 * p), &lmax, &lp, &lr);
+                }
             }
-#ifdef ORC_PLANT_WALL_SLOT /* planted mistake (tests only): the virtual state in the fluid cell's slot */
-            {
-                double *tmp = fluidRec;
-                fluidRec = virtRec;
-                virtRec = tmp;
-            }
-#endif
-            orc_reconstruct(s, fluidCell, local, U, fluidRec);
-            orc_eval_bc_values(bc, local, &s->face_info[ORC_NF * f], fluidRec, virtRec);
+            base += lim * (m->ncells / m->n[a]);
         }
-
-        double F[ORC_NF];
-        double faceMaxEig;
-        /* euler.cpp:232: the flux uses the owner-outward normal */
-        orc_eval_splitting(ownerRec, neighRec, normal, F, &faceMaxEig, &s->neg_p, &s->neg_rho);
-
-        *max_eig = (faceMaxEig < *max_eig) ? *max_eig : faceMaxEig; /* std::max(faceMaxEig, *maxEig) */
-
-#ifdef ORC_PLANT_SOLID_UPDATE /* planted mistake (tests only): residuals of unsolved cells accumulated */
-        ownerSolved = 1;
-        neighSolved = neigh >= 0;
-#endif
-        if (ownerSolved)
-            for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * owner + k] -= area * F[k];
-        if (neighSolved)
-            for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * neigh + k] += area * F[k];
+#pragma omp critical
+        {
+            gmax = (lmax < gmax) ? gmax : lmax;
+            gp += lp;
+            gr += lr;
+        }
     }
+    s->neg_p += gp;
+    s->neg_rho += gr;
+    *max_eig = gmax;
     return 0;
 }
 
 /* main.cpp:381-506: one SSP-RK3 step written inline in the reference. */
-double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
-                    double cfl, double t, double tmax, double *stage_eig)
+static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double cfl, double t, double tmax,
+                      double *stage_eig, int nthreads)
 {
     const orc_mesh *m = &s->mesh;
     const double minCellSize = m->h; /* main.cpp:351-356 */
     const double V = m->volume;
     double maxEig;
 
-    orc_compute_polynomials(s, U);
-    orc_compute_rhs(s, U, RHS, &maxEig);
+    orc_polys(s, U, nthreads);
+    if (nthreads > 1) orc_compute_rhs_threads(s, U, RHS, &maxEig, nthreads);
+    else orc_compute_rhs(s, U, RHS, &maxEig);
     if (stage_eig) stage_eig[0] = maxEig;
 
 #ifdef ORC_PLANT_DT /* planted mistake (tests only): the 0.9 safety factor dropped */
@@ -746,6 +820,7 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
 #endif
     if (t + dt > tmax) dt = tmax - t;             /* main.cpp:400-402 */
 
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:409-423 */
 #ifndef ORC_PLANT_SOLID_UPDATE
         if (!s->solved[c]) continue;
@@ -754,10 +829,12 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
             W[ORC_NF * c + k] = U[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V;
     }
 
-    orc_compute_polynomials(s, W);
-    orc_compute_rhs(s, W, RHS, &maxEig);
+    orc_polys(s, W, nthreads);
+    if (nthreads > 1) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
+    else orc_compute_rhs(s, W, RHS, &maxEig);
     if (stage_eig) stage_eig[1] = maxEig;
 
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:445-459 */
 #ifndef ORC_PLANT_SOLID_UPDATE
         if (!s->solved[c]) continue;
@@ -770,10 +847,12 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
 #endif
     }
 
-    orc_compute_polynomials(s, W);
-    orc_compute_rhs(s, W, RHS, &maxEig);
+    orc_polys(s, W, nthreads);
+    if (nthreads > 1) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
+    else orc_compute_rhs(s, W, RHS, &maxEig);
     if (stage_eig) stage_eig[2] = maxEig;
 
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (int64_t c = 0; c < m->ncells; ++c) {    /* main.cpp:481-495 */
 #ifndef ORC_PLANT_SOLID_UPDATE
         if (!s->solved[c]) continue;
@@ -782,6 +861,20 @@ double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
             U[ORC_NF * c + k] = (1. / 3) * U[ORC_NF * c + k] + (2. / 3) * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
     }
     return dt;
+}
+
+double orc_rk3_step(orc_solver *s, double *U, double *W, double *RHS,
+                    double cfl, double t, double tmax, double *stage_eig)
+{
+    return orc_rk3(s, U, W, RHS, cfl, t, tmax, stage_eig, 1);
+}
+
+/* The same step with the loops on nthreads threads (bit for bit the serial
+ * step; the CPU baseline on all host cores). */
+double orc_rk3_step_threads(orc_solver *s, double *U, double *W, double *RHS,
+                            double cfl, double t, double tmax, double *stage_eig, int nthreads)
+{
+    return orc_rk3(s, U, W, RHS, cfl, t, tmax, stage_eig, nthreads < 1 ? 1 : nthreads);
 }
 
 /* main.cpp:374-506 time loop */

@@ -149,6 +149,14 @@ int orc_compute_rhs(orc_solver *s, const double *U, double *RHS, double *max_eig
 /* One full SSP-RK3 step; returns dt used.  work: [ncells][5]. */
 double orc_rk3_step(orc_solver *s, double *U, double *work, double *RHS,
                     double cfl, double t, double tmax, double *stage_eig);
+
+/* Threaded forms (OpenMP), bit for bit the serial ones: the CPU baseline on
+ * all host cores.  The face loop runs axis by axis over groups of faces whose
+ * cells are disjoint, each group in increasing position, so every cell adds
+ * its faces in the serial order. */
+int orc_compute_rhs_threads(orc_solver *s, const double *U, double *RHS, double *max_eig, int nthreads);
+double orc_rk3_step_threads(orc_solver *s, double *U, double *work, double *RHS,
+                            double cfl, double t, double tmax, double *stage_eig, int nthreads);
 /* Loop until t >= tmax; returns number of steps, final t in *t_out. */
 int64_t orc_run(orc_solver *s, double *U, double cfl, double t0, double tmax,
                 int64_t max_steps, double *t_out);

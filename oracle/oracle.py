@@ -45,7 +45,7 @@ def build_variant(defines, outdir):
     """Build a copy of the oracle with extra -D flags (used to plant a mistake)."""
     os.makedirs(outdir, exist_ok=True)
     out = os.path.join(outdir, "liboracle_%s.so" % "_".join(d.lower() for d in defines))
-    cmd = ["gcc", "-O2", "-std=gnu99", "-fPIC", "-ffp-contract=off", "-shared"]
+    cmd = ["gcc", "-O3", "-std=gnu99", "-fPIC", "-ffp-contract=off", "-fopenmp", "-shared"]
     cmd += ["-D" + d for d in defines]
     cmd += ["-o", out, os.path.join(HERE, "mmf_oracle.c"), "-lm"]
     subprocess.run(cmd, check=True)
@@ -111,6 +111,10 @@ def _load(path):
     L.orc_compute_rhs.restype = C.c_int
     L.orc_rk3_step.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
     L.orc_rk3_step.restype = C.c_double
+    L.orc_rk3_step_threads.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp, C.c_int]
+    L.orc_rk3_step_threads.restype = C.c_double
+    L.orc_compute_rhs_threads.argtypes = [C.c_void_p, dp, dp, dp, C.c_int]
+    L.orc_compute_rhs_threads.restype = C.c_int
     L.orc_run.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int64, dp]
     L.orc_run.restype = C.c_int64
     L.orc_error.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
@@ -251,12 +255,17 @@ class Solver:
         lib().orc_solver_counters(self.s, C.byref(a), C.byref(b))
         return a.value, b.value
 
-    def compute_rhs(self, U):
+    def compute_rhs(self, U, threads=1):
+        """Residual and maximum eigenvalue; threads > 1 runs the threaded face
+        loop (bit for bit the serial one)."""
         U = _f64(U)
         R = np.empty((self.ncells, NF))
         me = C.c_double()
         lib().orc_compute_polynomials(self.s, _dp(U))
-        lib().orc_compute_rhs(self.s, _dp(U), _dp(R), C.byref(me))
+        if threads > 1:
+            lib().orc_compute_rhs_threads(self.s, _dp(U), _dp(R), C.byref(me), int(threads))
+        else:
+            lib().orc_compute_rhs(self.s, _dp(U), _dp(R), C.byref(me))
         return R, me.value
 
     def face_states(self, U):
@@ -276,12 +285,17 @@ class Solver:
                 out[c, f] = v
         return out
 
-    def rk3_step(self, U, W, cfl, t, tmax):
-        """In-place step of U (and work W); returns (dt, stage max eigenvalues)."""
+    def rk3_step(self, U, W, cfl, t, tmax, threads=1):
+        """In-place step of U (and work W); returns (dt, stage max eigenvalues).
+        threads > 1: the threaded step (bit for bit the serial one)."""
         R = np.empty((self.ncells, NF))
         eig = np.empty(3)
-        dt = lib().orc_rk3_step(self.s, _dp(U), _dp(W), _dp(R), float(cfl), float(t),
-                                float(tmax), _dp(eig))
+        if threads > 1:
+            dt = lib().orc_rk3_step_threads(self.s, _dp(U), _dp(W), _dp(R), float(cfl), float(t),
+                                            float(tmax), _dp(eig), int(threads))
+        else:
+            dt = lib().orc_rk3_step(self.s, _dp(U), _dp(W), _dp(R), float(cfl), float(t),
+                                    float(tmax), _dp(eig))
         return dt, eig
 
     def run(self, U, cfl, t0, tmax, max_steps=-1):

@@ -7,7 +7,7 @@ import sys
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-KEYS = {"value", "unit", "cores", "kind", "sample"}
+KEYS = {"value", "unit", "cores", "kind", "sample", "single_thread"}
 
 
 def run_bench(args, env=None):
@@ -27,9 +27,16 @@ def test_more_than_one_gpu_is_refused():
 def test_cpu_baseline_records(orc):
     sys.path.insert(0, ROOT)
     import bench
-    rec = bench.cpu_baseline(16, 1)
-    assert set(rec) == KEYS and rec["kind"] == "port" and rec["cores"] == 1 and rec["value"] > 0
-    assert rec["unit"] == "cell-updates/s" and "16^3" in rec["sample"]
+    rec = bench.cpu_baseline(16, 1, budget_mt=0.2)
+    assert set(rec) == KEYS and rec["kind"] == "port" and rec["cores"] == bench.cpu_threads() and rec["value"] > 0
+    assert rec["unit"] == "cell-updates/s" and "16^3" in rec["sample"] and "threads" in rec["sample"]
+    assert rec["single_thread"]["cores"] == 1 and rec["single_thread"]["value"] > 0
+    # OMP_NUM_THREADS, when set, is the thread count used
+    os.environ["OMP_NUM_THREADS"] = "3"
+    try:
+        assert bench.cpu_threads() == 3
+    finally:
+        del os.environ["OMP_NUM_THREADS"]
 
     class Flags:  # the sphere's fluid flags at 24^3 (centroid outside r = 0.25)
         def cell_flags(self):
@@ -38,6 +45,6 @@ def test_cpu_baseline_records(orc):
             z, y, x = np.meshgrid(c, c, c, indexing="ij")
             return (np.sqrt(x * x + y * y + z * z).reshape(-1) > 0.25).astype(np.uint8)
 
-    rec = bench.cpu_baseline_sphere(Flags(), 24, budget_s=0.2)
+    rec = bench.cpu_baseline_sphere(Flags(), 24, budget_s=0.2, budget_mt=0.2)
     assert set(rec) == KEYS and rec["kind"] == "port" and rec["value"] > 0
     assert "24^3" in rec["sample"]

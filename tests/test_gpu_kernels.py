@@ -173,3 +173,34 @@ def test_tile_kernel_refused_where_it_does_not_apply(mmf):
     s3 = mmf.Solver(3, [8, 8, 8], [-1.0, -1.0, -1.0], 0.25, [0] * 6, None, 2)
     with pytest.raises(mmf.MMFError):
         s3.set_kernel(mmf.KERNEL_TILE)
+
+
+def test_bodies_changed_between_graph_replayed_steps(mmf):
+    """Flags changed on a live context after graph-captured steps: a body moved
+    (the tile table rewritten under the same pointer), removed (no flags at all)
+    and set again; every phase bit-identical to the gather kernel."""
+    info = np.zeros((6, 5))
+    info[:] = [1.0, 0.3, -0.1, 0.05, 1.0]
+    bodies = [{"boxes": [[-0.4, -0.3, -0.5, 0.1, 0.35, 0.2]]}, {"spheres": [[0.2, 0.1, -0.1, 0.3]]}, None,
+              {"boxes": [[0.5, -0.9, -0.9, 0.8, -0.5, 0.0]]}]
+    out = {}
+    for kind in (mmf.KERNEL_GATHER, mmf.KERNEL_MARCH):
+        s, U = build(mmf, 3, [64, 24, 20], [3, 0, 0, 3, 0, 3], info)
+        s.set_state(U)
+        s.set_time(0.0, 10.0, 0.5)
+        res = []
+        for b in bodies:
+            if b is None:
+                s.set_cell_flags(np.ones(s.ncells, dtype=np.uint8))
+            else:
+                s.set_bodies(**b)
+            s.set_kernel(kind, 8, 8)
+            assert s.kernel()[0] == kind
+            s.advance(17)  # one 16-step graph and one 1-step graph
+            res.append((s.get_state(), s.time()[:3], s.cell_flags()))
+        out[kind] = res
+        s.close()
+    for (a, ta, fa), (b, tb, fb) in zip(out[mmf.KERNEL_GATHER], out[mmf.KERNEL_MARCH]):
+        assert np.array_equal(fa, fb)
+        assert ta == tb
+        assert np.array_equal(a, b)

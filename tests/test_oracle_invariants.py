@@ -252,3 +252,55 @@ def test_planted_mistakes_are_caught(orc, tmp_path, plant, case):
     assert abs(l1 - float(expected)) / float(expected) > 1e-8, (plant, l1)
     l1, _, _, _, _ = orc.run_reference_case(orc.PROBLEM_KEYS[key], dim, nc, tend)
     assert "%.12e" % l1 == expected
+
+
+THREAD_CASES = [
+    ("3d free", 3, [21, 13, 17], [0] * 6, None, 1),
+    ("3d periodic box", 3, [70, 9, 12], [4, 4, 1, 2, 4, 4], [[0.5, 0.5, 0.5, 1.6, 1.2, 2.0]], 1),
+    ("3d order 2 dirichlet", 3, [19, 22, 8], [3, 0, 1, 0, 0, 3], [[1.0, 1.0, -1.0, 2.0, 3.0, 9.0]], 2),
+    ("2d ragged", 2, [131, 7], [4, 4, 0, 1], None, 1),
+    ("2d order 2 body", 2, [66, 40], [1, 0, 3, 4][:2] + [0, 1], [[2.0, 2.0, -1.0, 4.0, 3.0, 1.0]], 2),
+]
+
+
+@pytest.mark.parametrize("name,dim,n,bc,boxes,order", THREAD_CASES, ids=[c[0] for c in THREAD_CASES])
+def test_threaded_steps_equal_serial(orc, name, dim, n, bc, boxes, order):
+    """The threaded face loop and RK update (the all-cores CPU baseline) give
+    the serial oracle's residual, eigenvalue maximum, counters and steps bit
+    for bit, for 1, 3 and 8 threads."""
+    import ctypes as C
+    pb = orc.problem(orc.RADSOD, dim)
+    h = 0.25
+    info = np.tile([0.9, 0.3, -0.2, 0.1 if dim == 3 else 0.0, 1.1], (6, 1))
+    probe = orc.Solver(pb, order, n, [0.0] * 3, h, side_bc=bc, side_info=info)
+    fluid = None
+    if boxes:
+        fluid = np.empty(probe.ncells, dtype=np.uint8)
+        b = np.ascontiguousarray(boxes, dtype=np.float64)
+        orc.lib().orc_mark_fluid(C.c_void_p(probe.s), b.ctypes.data_as(C.POINTER(C.c_double)), len(b), None, 0,
+                                 fluid.ctypes.data_as(C.POINTER(C.c_uint8)))
+        assert 0 < fluid.sum() < probe.ncells
+    rng = np.random.default_rng(len(name))
+    U0 = np.array([orc.prim2cons([rng.uniform(0.5, 2), *rng.uniform(-0.7, 0.7, 2),
+                                  rng.uniform(-0.7, 0.7) if dim == 3 else 0.0, rng.uniform(0.5, 2)])
+                   for _ in range(probe.ncells)])
+    Uneg = U0.copy()
+    Uneg[3] = [-0.5, 0.0, 0.0, 0.0, -1.0]  # counted negative density and pressure
+    out = {}
+    for threads in (1, 3, 8):
+        s = orc.Solver(pb, order, n, [0.0] * 3, h, side_bc=bc, side_info=info, fluid=fluid)
+        R, me = s.compute_rhs(Uneg, threads=threads)
+        # steps from a state without it (a NaN eigenvalue makes the serial maximum depend on the face order)
+        U, W = U0.copy(), U0.copy()
+        t, steps = 0.0, []
+        for _ in range(2):
+            dt, eig = s.rk3_step(U, W, 0.5, t, 1e30, threads=threads)
+            t += dt
+            steps.append((dt, tuple(eig)))
+        out[threads] = (R, me, s.counters(), steps, U)
+    ref = out[1]
+    assert ref[2][1] > 0
+    for threads in (3, 8):
+        got = out[threads]
+        assert np.array_equal(got[0], ref[0]) and got[1] == ref[1] and got[2] == ref[2]
+        assert got[3] == ref[3] and np.array_equal(got[4], ref[4])
