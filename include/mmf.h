@@ -123,6 +123,32 @@ int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw);
  * rhs_aos: host [ncells][5] or NULL; max_eig: host scalar or NULL. */
 int mmf_compute_rhs(mmf_ctx *ctx, double *rhs_aos, double *max_eig);
 
+/* ---- the same RHS, allocating nothing -------------------------------------- */
+/* No counterpart in the reference: a variant of mmf_compute_rhs for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  It creates no context and calls
+ * no hipMalloc / hipFree: the stage-maximum lines and positivity counters
+ * live in a library-owned device array of 12,672 bytes (its maximum size),
+ * one per GPU, held until the process ends.
+ *   desc      the grid (validated as in mmf_create; desc->order 1 or 2)
+ *   flags     device [ncells] MMF_CELL_* bits, or NULL (every cell fluid and
+ *             solved)
+ *   U         device SoA state [5][field_stride] (conservative variables)
+ *   rhs       device SoA residual [5][field_stride], written (not aliasing U)
+ *   max_eig   device double or NULL: the maximum face eigenvalue
+ *   counts    device int64[2] or NULL: negative pressure / density face sides
+ *   stream    a hipStream_t, or NULL for the null stream
+ * field_stride must be >= ncells and a multiple of 64.  Enqueues on 'stream'
+ * and returns without synchronising; results are bitwise those of
+ * mmf_compute_rhs on the same grid, flags and state.  Calls on one GPU share
+ * the array: calls that overlap on one GPU (unordered streams) race.  Errors
+ * (MMF_ERR_INVALID for a descriptor or stride beyond the limits) leave a
+ * message for mmf_noalloc_last_error() in the calling thread. */
+int mmf_compute_rhs_noalloc(const mmf_grid_desc *desc, const uint8_t *flags, const double *U,
+                            int64_t field_stride, double *rhs, double *max_eig, int64_t *counts,
+                            void *stream);
+const char *mmf_noalloc_last_error(void);
+
 /* ---- RK3 time loop (main.cpp:374-506) ------------------------------------ */
 /* dt = 0.9 * cfl * h / maxEig(stage 1), clipped to tmax (main.cpp:398-402). */
 int mmf_set_time(mmf_ctx *ctx, double t, double tmax, double cfl);

@@ -102,6 +102,8 @@ def lib():
         "mmf_run": ([vp, C.c_int64, i64p, dp], C.c_int),
         "mmf_get_time": ([vp, dp, dp, i64p, dp], C.c_int),
         "mmf_get_nonfinite_steps": ([vp, i64p], C.c_int),
+        "mmf_compute_rhs_noalloc": ([C.POINTER(GridDesc), vp, vp, C.c_int64, vp, vp, vp, vp], C.c_int),
+        "mmf_noalloc_last_error": ([], C.c_char_p),
         "mmf_synchronize": ([vp], C.c_int),
         "mmf_write_vtk": ([vp, C.c_char_p], C.c_int),
         "mmf_write_vtk_async": ([vp, C.c_char_p], C.c_int),
