@@ -392,6 +392,60 @@ int mmf_get_face_states(mmf_ctx *ctx, double *faces);
  * host AoS [ncells / factor^2][5] */
 int mmf_restrict_state(mmf_ctx *ctx, int32_t factor, double *coarse);
 
+/* ---- general mesh: an explicit interface list ---------------------------- */
+/* The general MeshGeometricalInfo (mesh_info.hpp:31-105, cached by _extract,
+ * mesh_info.cpp:86-118) handed over as arrays, for meshes the uniform
+ * descriptor cannot express (bitpit's octree with hanging faces, any
+ * polyhedral cell list).  Cells and faces are numbered by the caller (raw
+ * order); the interface loop of euler::computeRHS (euler.cpp:153-247) runs in
+ * the given face order, so each cell's residual is summed in that order and
+ * the results equal a serial interface loop over the same list bit for bit.
+ * Order 1 (reconstruction.cpp:90-97).  Borders: faces with neighbour -1 take
+ * face_bc (FREE_FLOW, REFLECTING, WALL or DIRICHLET with face_info
+ * primitives: problem::getBorderBCType / getBorderBCInfo evaluated by the
+ * caller); interior faces become walls where exactly one side is fluid
+ * (main.cpp:251-277).  dt = 0.9 cfl min_cell_size / maxEig (main.cpp:351-356,
+ * 398); the RK update divides by each cell's volume (main.cpp:409-495). */
+typedef struct mmf_mesh_desc {
+    int32_t dim;                 /* 2 or 3 (informational: normals carry 3 components) */
+    int64_t ncells;              /* < 2^31 */
+    int64_t nfaces;              /* < 2^30 */
+    const double *cell_volume;   /* [ncells] rawGetCellVolume */
+    double min_cell_size;        /* min over cells of rawGetCellSize */
+    const int64_t *face_owner;   /* [nfaces] Interface::getOwner as a cell index */
+    const int64_t *face_neigh;   /* [nfaces] Interface::getNeigh as a cell index, -1 on borders */
+    const double *face_area;     /* [nfaces] rawGetInterfaceArea */
+    const double *face_normal;   /* [nfaces][3] rawGetInterfaceNormal (owner-outward, unit) */
+    const int32_t *face_bc;      /* [nfaces] border faces' BC codes, or NULL (free flow) */
+    const double *face_info;     /* [nfaces][5] Dirichlet primitives (p,u,v,w,T), or NULL */
+} mmf_mesh_desc;
+
+typedef struct mmf_mesh mmf_mesh;
+
+int mmf_mesh_create(const mmf_mesh_desc *desc, mmf_mesh **out);
+int mmf_mesh_destroy(mmf_mesh *m);
+const char *mmf_mesh_last_error(const mmf_mesh *m);
+int mmf_mesh_sizes(const mmf_mesh *m, int64_t *ncells, int64_t *nfaces);
+/* fluid: [ncells] 0/1 or NULL; solved: NULL -> solved = fluid (main.cpp:222-237) */
+int mmf_mesh_set_cell_flags(mmf_mesh *m, const uint8_t *fluid, const uint8_t *solved);
+/* host AoS [ncells][5] conservative states */
+int mmf_mesh_set_state(mmf_mesh *m, const double *U_aos);
+int mmf_mesh_get_state(mmf_mesh *m, double *U_aos);
+/* euler::computeRHS (euler.cpp:127-249) of the current state */
+int mmf_mesh_compute_rhs(mmf_mesh *m, double *rhs_aos, double *max_eig);
+/* the RK3 time loop, as mmf_set_time / mmf_advance / mmf_run / mmf_get_time */
+int mmf_mesh_set_time(mmf_mesh *m, double t, double tmax, double cfl);
+int mmf_mesh_advance(mmf_mesh *m, int64_t nsteps);
+int mmf_mesh_set_step_graphs(mmf_mesh *m, int32_t on);
+int mmf_mesh_run(mmf_mesh *m, int64_t max_steps, int64_t *steps_done, double *t_final);
+int mmf_mesh_get_time(mmf_mesh *m, double *t, double *dt, int64_t *step, double *stage_eig);
+/* negative p / rho face sides (euler.cpp:94-101) and steps that met a
+ * non-finite state */
+int mmf_mesh_get_counters(mmf_mesh *m, int64_t *neg_p, int64_t *neg_rho, int64_t *nonfinite_steps);
+/* HIP-event timing of the face and cell kernels of every stage */
+int mmf_mesh_enable_timing(mmf_mesh *m, int32_t on);
+int mmf_mesh_read_timing(mmf_mesh *m, double *face_ms, double *cell_ms, int64_t *stages);
+
 /* ---- bandwidth calibration ------------------------------------------------ */
 /* Copies nbytes device-to-device with a plain vector-load/store kernel reps
  * times; returns the average kernel milliseconds (HIP events). */

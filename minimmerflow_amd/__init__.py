@@ -37,6 +37,14 @@ class MMFError(RuntimeError):
         super().__init__("mmf status %d (%s)%s" % (status, _status_string(status), (": " + msg) if msg else ""))
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("ncells", C.c_int64), ("nfaces", C.c_int64),
+                ("cell_volume", C.POINTER(C.c_double)), ("min_cell_size", C.c_double),
+                ("face_owner", C.POINTER(C.c_int64)), ("face_neigh", C.POINTER(C.c_int64)),
+                ("face_area", C.POINTER(C.c_double)), ("face_normal", C.POINTER(C.c_double)),
+                ("face_bc", C.POINTER(C.c_int32)), ("face_info", C.POINTER(C.c_double))]
+
+
 class GridDesc(C.Structure):
     _fields_ = [("dim", C.c_int32), ("order", C.c_int32), ("n", C.c_int64 * 3),
                 ("origin", C.c_double * 3), ("h", C.c_double), ("side_bc", C.c_int32 * 6),
@@ -165,6 +173,22 @@ def lib():
         "mmf_reset_learned_counts": ([vp], C.c_int),
         "mmf_get_face_states": ([vp, dp], C.c_int),
         "mmf_restrict_state": ([vp, C.c_int32, dp], C.c_int),
+        "mmf_mesh_create": ([C.POINTER(MeshDesc), C.POINTER(C.c_void_p)], C.c_int),
+        "mmf_mesh_destroy": ([vp], C.c_int),
+        "mmf_mesh_last_error": ([vp], C.c_char_p),
+        "mmf_mesh_sizes": ([vp, i64p, i64p], C.c_int),
+        "mmf_mesh_set_cell_flags": ([vp, u8p, u8p], C.c_int),
+        "mmf_mesh_set_state": ([vp, dp], C.c_int),
+        "mmf_mesh_get_state": ([vp, dp], C.c_int),
+        "mmf_mesh_compute_rhs": ([vp, dp, dp], C.c_int),
+        "mmf_mesh_set_time": ([vp, C.c_double, C.c_double, C.c_double], C.c_int),
+        "mmf_mesh_advance": ([vp, C.c_int64], C.c_int),
+        "mmf_mesh_set_step_graphs": ([vp, C.c_int32], C.c_int),
+        "mmf_mesh_run": ([vp, C.c_int64, i64p, dp], C.c_int),
+        "mmf_mesh_get_time": ([vp, dp, dp, i64p, dp], C.c_int),
+        "mmf_mesh_get_counters": ([vp, i64p, i64p, i64p], C.c_int),
+        "mmf_mesh_enable_timing": ([vp, C.c_int32], C.c_int),
+        "mmf_mesh_read_timing": ([vp, dp, dp, i64p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -728,3 +752,106 @@ class Learner:
         y = np.empty((x.shape[0], LEARN_NOUT), dtype=np.float32)
         self._c(lib().mmf_learn_forward(self.h, _fp(x), x.shape[0], 1 if mfma else 0, _fp(y)))
         return y
+
+
+# ---- general mesh: an explicit interface list ------------------------------------
+
+class Mesh:
+    """Device-resident solver of an explicit face list (the general
+    MeshGeometricalInfo); fm is a meshes.FaceMesh or anything with its arrays."""
+
+    def __init__(self, fm):
+        d = MeshDesc()
+        self._keep = [np.ascontiguousarray(fm.volume, dtype=np.float64),
+                      np.ascontiguousarray(fm.owner, dtype=np.int64), np.ascontiguousarray(fm.neigh, dtype=np.int64),
+                      np.ascontiguousarray(fm.area, dtype=np.float64),
+                      np.ascontiguousarray(fm.normal, dtype=np.float64).reshape(-1, 3),
+                      np.ascontiguousarray(np.maximum(fm.face_bc, 0), dtype=np.int32),
+                      np.ascontiguousarray(fm.face_info, dtype=np.float64).reshape(-1, NF)]
+        vol, own, nei, ar, nrm, fbc, finfo = self._keep
+        d.dim = int(fm.dim)
+        d.ncells = len(vol)
+        d.nfaces = len(own)
+        d.cell_volume = _dp(vol)
+        d.min_cell_size = float(fm.min_cell_size)
+        d.face_owner = own.ctypes.data_as(C.POINTER(C.c_int64))
+        d.face_neigh = nei.ctypes.data_as(C.POINTER(C.c_int64))
+        d.face_area = _dp(ar)
+        d.face_normal = _dp(nrm)
+        d.face_bc = fbc.ctypes.data_as(C.POINTER(C.c_int32))
+        d.face_info = _dp(finfo)
+        h = C.c_void_p()
+        _check(lib().mmf_mesh_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.ncells, self.nfaces = d.ncells, d.nfaces
+        self.dim = d.dim
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mmf_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        if rc != OK:
+            m = lib().mmf_mesh_last_error(self.h)
+            raise MMFError(rc, m.decode() if m else "")
+
+    def set_cell_flags(self, fluid, solved=None):
+        f = np.ascontiguousarray(fluid, dtype=np.uint8)
+        s = None if solved is None else np.ascontiguousarray(solved, dtype=np.uint8)
+        self._c(lib().mmf_mesh_set_cell_flags(self.h, f.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                              None if s is None else s.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def set_state(self, U):
+        U = _f64(U).reshape(self.ncells, NF)
+        self._c(lib().mmf_mesh_set_state(self.h, _dp(U)))
+
+    def get_state(self):
+        U = np.empty((self.ncells, NF))
+        self._c(lib().mmf_mesh_get_state(self.h, _dp(U)))
+        return U
+
+    def compute_rhs(self):
+        R = np.empty((self.ncells, NF))
+        me = C.c_double()
+        self._c(lib().mmf_mesh_compute_rhs(self.h, _dp(R), C.byref(me)))
+        return R, me.value
+
+    def set_time(self, t, tmax, cfl):
+        self._c(lib().mmf_mesh_set_time(self.h, float(t), float(tmax), float(cfl)))
+
+    def advance(self, nsteps):
+        self._c(lib().mmf_mesh_advance(self.h, int(nsteps)))
+
+    def set_step_graphs(self, on=True):
+        self._c(lib().mmf_mesh_set_step_graphs(self.h, 1 if on else 0))
+
+    def run(self, max_steps=-1):
+        steps, t = C.c_int64(), C.c_double()
+        self._c(lib().mmf_mesh_run(self.h, int(max_steps), C.byref(steps), C.byref(t)))
+        return steps.value, t.value
+
+    def time(self):
+        t, dt, step = C.c_double(), C.c_double(), C.c_int64()
+        eig = np.empty(3)
+        self._c(lib().mmf_mesh_get_time(self.h, C.byref(t), C.byref(dt), C.byref(step), _dp(eig)))
+        return t.value, dt.value, step.value, eig
+
+    def counters(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._c(lib().mmf_mesh_get_counters(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def enable_timing(self, on=True):
+        self._c(lib().mmf_mesh_enable_timing(self.h, 1 if on else 0))
+
+    def read_timing(self):
+        f, c, n = C.c_double(), C.c_double(), C.c_int64()
+        self._c(lib().mmf_mesh_read_timing(self.h, C.byref(f), C.byref(c), C.byref(n)))
+        return f.value, c.value, n.value

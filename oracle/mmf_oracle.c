@@ -27,6 +27,9 @@
 /* constants.hpp:55 */
 static const double ORC_GAMMA = 1.4;
 
+/* main.cpp:417 rawGetCellVolume: per cell on an explicit mesh, else uniform */
+#define ORC_VOL(m, c) ((m)->cell_volume ? (m)->cell_volume[(c)] : (m)->volume)
+
 /* ------------------------------------------------------------------ */
 /* utils                                                               */
 /* ------------------------------------------------------------------ */
@@ -452,6 +455,8 @@ void orc_mesh_free(orc_mesh *m)
     free(m->face_centroid);
     free(m->face_axis);
     free(m->face_side);
+    free(m->face_area);
+    free(m->cell_volume);
     memset(m, 0, sizeof(*m));
 }
 
@@ -540,6 +545,69 @@ int orc_solver_init(orc_solver *s, const orc_problem *pb, int order,
 #else
             s->face_bc[f] = (of ^ nf) ? ORC_BC_WALL : ORC_BC_NONE;
 #endif
+        }
+    }
+    return 0;
+}
+
+/* A solver on an explicit interface list, the general MeshGeometricalInfo
+ * (mesh_info.cpp:86-118): owner / neighbour (-1 on borders) / area /
+ * owner-outward normal per face, volume (and centroid) per cell, the border
+ * faces' BC codes and Dirichlet data, min_cell_size (main.cpp:351-356).
+ * Interior faces become walls where exactly one side is fluid
+ * (main.cpp:251-277).  Order 1 (the reference's reconstruction).  The face
+ * loop runs in the given face order. */
+int orc_solver_init_faces(orc_solver *s, const orc_problem *pb, int dim, int64_t ncells, int64_t nfaces,
+                          const double *cell_volume, const double *cell_centroid, double min_cell_size,
+                          const int64_t *owner, const int64_t *neigh, const double *area, const double *normal,
+                          const int *border_bc, const double *border_info, const uint8_t *fluid)
+{
+    memset(s, 0, sizeof(*s));
+    s->pb = *pb;
+    s->order = 1;
+    orc_mesh *m = &s->mesh;
+    m->dim = dim;
+    m->n[0] = ncells;
+    m->n[1] = m->n[2] = 1;
+    m->h = min_cell_size;
+    m->ncells = ncells;
+    m->nfaces = nfaces;
+    m->cell_centroid = (double *)calloc((size_t)ncells * 3, sizeof(double));
+    m->cell_volume = (double *)malloc(sizeof(double) * ncells);
+    m->face_owner = (int64_t *)malloc(sizeof(int64_t) * nfaces);
+    m->face_neigh = (int64_t *)malloc(sizeof(int64_t) * nfaces);
+    m->face_normal = (double *)malloc(sizeof(double) * 3 * nfaces);
+    m->face_centroid = (double *)calloc((size_t)nfaces * 3, sizeof(double));
+    m->face_axis = (int *)calloc((size_t)nfaces, sizeof(int));
+    m->face_side = (int *)malloc(sizeof(int) * nfaces);
+    m->face_area = (double *)malloc(sizeof(double) * nfaces);
+    s->fluid = (uint8_t *)malloc(ncells);
+    s->solved = (uint8_t *)malloc(ncells);
+    s->face_bc = (int *)malloc(sizeof(int) * nfaces);
+    s->face_info = (double *)calloc((size_t)nfaces * ORC_NF, sizeof(double));
+    if (!m->cell_centroid || !m->cell_volume || !m->face_owner || !m->face_neigh || !m->face_normal ||
+        !m->face_centroid || !m->face_axis || !m->face_side || !m->face_area || !s->fluid || !s->solved ||
+        !s->face_bc || !s->face_info) {
+        orc_solver_free(s);
+        return -1;
+    }
+    memcpy(m->cell_volume, cell_volume, sizeof(double) * ncells);
+    if (cell_centroid) memcpy(m->cell_centroid, cell_centroid, sizeof(double) * 3 * ncells);
+    memcpy(m->face_owner, owner, sizeof(int64_t) * nfaces);
+    memcpy(m->face_neigh, neigh, sizeof(int64_t) * nfaces);
+    memcpy(m->face_area, area, sizeof(double) * nfaces);
+    memcpy(m->face_normal, normal, sizeof(double) * 3 * nfaces);
+    for (int64_t c = 0; c < ncells; ++c) {
+        s->fluid[c] = fluid ? fluid[c] : 1;
+        s->solved[c] = s->fluid[c];
+    }
+    for (int64_t f = 0; f < nfaces; ++f) {
+        m->face_side[f] = -1;
+        if (neigh[f] < 0) {
+            s->face_bc[f] = border_bc ? border_bc[f] : ORC_BC_FREE_FLOW;
+            if (border_info) memcpy(&s->face_info[ORC_NF * f], &border_info[ORC_NF * f], sizeof(double) * ORC_NF);
+        } else {
+            s->face_bc[f] = (s->fluid[owner[f]] ^ s->fluid[neigh[f]]) ? ORC_BC_WALL : ORC_BC_NONE;
         }
     }
     return 0;
@@ -671,7 +739,7 @@ static void orc_face(const orc_solver *s, const double *U, double *RHS, int64_t 
 #endif
 
     int bc = s->face_bc[f];
-    const double area = m->area;
+    const double area = m->face_area ? m->face_area[f] : m->area; /* rawGetInterfaceArea */
     const double *normal = &m->face_normal[3 * f];
 
     double ownerRec[ORC_NF], neighRec[ORC_NF];
@@ -805,11 +873,10 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
 {
     const orc_mesh *m = &s->mesh;
     const double minCellSize = m->h; /* main.cpp:351-356 */
-    const double V = m->volume;
     double maxEig;
 
     orc_polys(s, U, nthreads);
-    if (nthreads > 1) orc_compute_rhs_threads(s, U, RHS, &maxEig, nthreads);
+    if (nthreads > 1 && !m->face_area) orc_compute_rhs_threads(s, U, RHS, &maxEig, nthreads);
     else orc_compute_rhs(s, U, RHS, &maxEig);
     if (stage_eig) stage_eig[0] = maxEig;
 
@@ -826,11 +893,11 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
         if (!s->solved[c]) continue;
 #endif
         for (int k = 0; k < ORC_NF; ++k)
-            W[ORC_NF * c + k] = U[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V;
+            W[ORC_NF * c + k] = U[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / ORC_VOL(m, c);
     }
 
     orc_polys(s, W, nthreads);
-    if (nthreads > 1) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
+    if (nthreads > 1 && !m->face_area) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
     else orc_compute_rhs(s, W, RHS, &maxEig);
     if (stage_eig) stage_eig[1] = maxEig;
 
@@ -841,14 +908,14 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
 #endif
         for (int k = 0; k < ORC_NF; ++k)
 #ifdef ORC_PLANT_RK /* planted mistake (tests only): stage-2 weights of a different scheme */
-            W[ORC_NF * c + k] = 0.5 * U[ORC_NF * c + k] + 0.5 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
+            W[ORC_NF * c + k] = 0.5 * U[ORC_NF * c + k] + 0.5 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / ORC_VOL(m, c));
 #else
-            W[ORC_NF * c + k] = 0.75 * U[ORC_NF * c + k] + 0.25 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
+            W[ORC_NF * c + k] = 0.75 * U[ORC_NF * c + k] + 0.25 * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / ORC_VOL(m, c));
 #endif
     }
 
     orc_polys(s, W, nthreads);
-    if (nthreads > 1) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
+    if (nthreads > 1 && !m->face_area) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
     else orc_compute_rhs(s, W, RHS, &maxEig);
     if (stage_eig) stage_eig[2] = maxEig;
 
@@ -858,7 +925,7 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
         if (!s->solved[c]) continue;
 #endif
         for (int k = 0; k < ORC_NF; ++k)
-            U[ORC_NF * c + k] = (1. / 3) * U[ORC_NF * c + k] + (2. / 3) * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / V);
+            U[ORC_NF * c + k] = (1. / 3) * U[ORC_NF * c + k] + (2. / 3) * (W[ORC_NF * c + k] + dt * RHS[ORC_NF * c + k] / ORC_VOL(m, c));
     }
     return dt;
 }
@@ -913,8 +980,8 @@ void orc_error(const orc_solver *s, const double *U, double t, double *l1, doubl
     for (int64_t c = 0; c < s->mesh.ncells; ++c) {
         orc_exact_conservatives(&s->pb, &s->mesh.cell_centroid[3 * c], t, ex);
         double d = U[ORC_NF * c + 0] - ex[0];
-        e1 += fabs(d) * s->mesh.volume;
-        e2 += d * d * s->mesh.volume;
+        e1 += fabs(d) * ORC_VOL(&s->mesh, c);
+        e2 += d * d * ORC_VOL(&s->mesh, c);
     }
     if (l1) *l1 = e1;
     if (l2) *l2 = sqrt(e2);
@@ -950,4 +1017,19 @@ void orc_solver_flags(const orc_solver *s, uint8_t *fluid, uint8_t *solved)
 void orc_solver_face_bc(const orc_solver *s, int *out)
 {
     memcpy(out, s->face_bc, sizeof(int) * s->mesh.nfaces);
+}
+
+/* The solver's face list (owner, neighbour, owner-outward normal, area) and
+ * cell volumes: what an explicit-mesh caller passes for the same grid. */
+void orc_solver_face_list(const orc_solver *s, int64_t *owner, int64_t *neigh, double *normal, double *area)
+{
+    const orc_mesh *m = &s->mesh;
+    memcpy(owner, m->face_owner, sizeof(int64_t) * m->nfaces);
+    memcpy(neigh, m->face_neigh, sizeof(int64_t) * m->nfaces);
+    memcpy(normal, m->face_normal, sizeof(double) * 3 * m->nfaces);
+    for (int64_t f = 0; f < m->nfaces; ++f) area[f] = m->face_area ? m->face_area[f] : m->area;
+}
+void orc_solver_cell_volumes(const orc_solver *s, double *vol)
+{
+    for (int64_t c = 0; c < s->mesh.ncells; ++c) vol[c] = ORC_VOL(&s->mesh, c);
 }

@@ -112,6 +112,8 @@ typedef struct orc_mesh {
     double *face_centroid;       /* [nfaces][3] */
     int *face_axis;              /* [nfaces] */
     int *face_side;              /* [nfaces]: border side 0..5 (x-,x+,y-,y+,z-,z+), -1 interior */
+    double *face_area;           /* [nfaces] on an explicit mesh, else NULL (uniform 'area') */
+    double *cell_volume;         /* [ncells] on an explicit mesh, else NULL (uniform 'volume') */
 } orc_mesh;
 
 int orc_mesh_build(orc_mesh *m, int dim, const int64_t *n, const double *origin, double h,
@@ -139,6 +141,17 @@ int orc_solver_init(orc_solver *s, const orc_problem *pb, int order,
                     const int64_t *n, const double *origin, double h,
                     const int *side_bc, const double *side_info, const uint8_t *fluid);
 void orc_solver_free(orc_solver *s);
+
+/* A solver on an explicit interface list (the general MeshGeometricalInfo,
+ * mesh_info.cpp:86-118): per face owner, neighbour (-1 on borders), area,
+ * owner-outward normal [3]; per cell volume (and centroid [3] or NULL);
+ * border faces' BC codes and Dirichlet primitives [5] (NULL: free flow / no
+ * data); min_cell_size for the dt rule.  Order 1; the face loop runs in the
+ * given order. */
+int orc_solver_init_faces(orc_solver *s, const orc_problem *pb, int dim, int64_t ncells, int64_t nfaces,
+                          const double *cell_volume, const double *cell_centroid, double min_cell_size,
+                          const int64_t *owner, const int64_t *neigh, const double *area, const double *normal,
+                          const int *border_bc, const double *border_info, const uint8_t *fluid);
 
 void orc_initial_state(const orc_solver *s, double *U);
 void orc_compute_polynomials(orc_solver *s, const double *U);
@@ -178,6 +191,8 @@ void orc_solver_counters(const orc_solver *s, int64_t *neg_p, int64_t *neg_rho);
 void orc_solver_cell_centroids(const orc_solver *s, double *out);
 void orc_solver_flags(const orc_solver *s, uint8_t *fluid, uint8_t *solved);
 void orc_solver_face_bc(const orc_solver *s, int *out);
+void orc_solver_face_list(const orc_solver *s, int64_t *owner, int64_t *neigh, double *normal, double *area);
+void orc_solver_cell_volumes(const orc_solver *s, double *vol);
 
 #ifdef __cplusplus
 }

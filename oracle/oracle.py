@@ -120,6 +120,12 @@ def _load(path):
     L.orc_error.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
     L.orc_reconstruct.argtypes = [C.c_void_p, C.c_int64, dp, dp, dp]
     L.orc_mark_fluid.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int, C.POINTER(C.c_uint8)]
+    L.orc_solver_init_faces.argtypes = [C.c_void_p, C.POINTER(Problem), C.c_int, C.c_int64, C.c_int64, dp, dp,
+                                        C.c_double, i64p, i64p, dp, dp, C.POINTER(C.c_int), dp,
+                                        C.POINTER(C.c_uint8)]
+    L.orc_solver_init_faces.restype = C.c_int
+    L.orc_solver_face_list.argtypes = [C.c_void_p, i64p, i64p, dp, dp]
+    L.orc_solver_cell_volumes.argtypes = [C.c_void_p, dp]
     _lib = L
     return L
 
@@ -222,6 +228,54 @@ class Solver:
             raise ValueError("oracle solver init failed (%d)" % rc)
         self.ncells = L.orc_solver_ncells(self.s)
         self.nfaces = L.orc_solver_nfaces(self.s)
+
+    @classmethod
+    def from_faces(cls, pb, dim, volume, owner, neigh, area, normal, min_cell_size, border_bc=None,
+                   border_info=None, fluid=None, centroids=None):
+        """Oracle solver on an explicit interface list (order 1): the general
+        MeshGeometricalInfo; the face loop runs in the given face order."""
+        L = lib()
+        self = cls.__new__(cls)
+        self.pb = pb
+        self.dim = dim
+        self._keep = [np.ascontiguousarray(volume, dtype=np.float64),
+                      np.ascontiguousarray(owner, dtype=np.int64), np.ascontiguousarray(neigh, dtype=np.int64),
+                      np.ascontiguousarray(area, dtype=np.float64),
+                      np.ascontiguousarray(normal, dtype=np.float64).reshape(-1, 3),
+                      None if border_bc is None else np.ascontiguousarray(border_bc, dtype=np.int32),
+                      None if border_info is None else np.ascontiguousarray(border_info, dtype=np.float64),
+                      None if fluid is None else np.ascontiguousarray(fluid, dtype=np.uint8),
+                      None if centroids is None else np.ascontiguousarray(centroids, dtype=np.float64)]
+        vol, own, nei, ar, nrm, bbc, binfo, fl, cen = self._keep
+        self.h = float(min_cell_size)
+        self.s = L.orc_solver_alloc()
+        rc = L.orc_solver_init_faces(
+            self.s, C.byref(pb), int(dim), len(vol), len(own), _dp(vol), None if cen is None else _dp(cen),
+            self.h, own.ctypes.data_as(C.POINTER(C.c_int64)), nei.ctypes.data_as(C.POINTER(C.c_int64)), _dp(ar),
+            _dp(nrm), None if bbc is None else bbc.ctypes.data_as(C.POINTER(C.c_int)),
+            None if binfo is None else _dp(binfo), None if fl is None else fl.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            L.orc_solver_dealloc(self.s)
+            self.s = None
+            raise ValueError("oracle face-list solver init failed (%d)" % rc)
+        self.ncells = L.orc_solver_ncells(self.s)
+        self.nfaces = L.orc_solver_nfaces(self.s)
+        return self
+
+    def face_list(self):
+        """(owner, neighbour, normal [nfaces][3], area) of this solver's faces."""
+        o = np.empty(self.nfaces, dtype=np.int64)
+        n = np.empty(self.nfaces, dtype=np.int64)
+        nrm = np.empty((self.nfaces, 3))
+        a = np.empty(self.nfaces)
+        lib().orc_solver_face_list(self.s, o.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   n.ctypes.data_as(C.POINTER(C.c_int64)), _dp(nrm), _dp(a))
+        return o, n, nrm, a
+
+    def cell_volumes(self):
+        v = np.empty(self.ncells)
+        lib().orc_solver_cell_volumes(self.s, _dp(v))
+        return v
 
     def __del__(self):
         if getattr(self, "s", None):
