@@ -9,7 +9,9 @@ FP64.  --workload sphere192 runs the immersed-sphere case instead (192^3 on
 free-flow elsewhere); --workload batch512 runs 512 independent seeded 2D
 cases together (persistent stage kernel, graph-replayed step); --workload
 learned128 trains the learned face reconstruction on the device and compares
-its L1 density error and wall time with the plain limiter.
+its L1 density error and wall time with the plain limiter; --workload
+octree128 runs radial Sod on a 2:1 refined octree mesh through the explicit
+interface-list kernels.
 
 A step is one SSP-RK3 time step (three fused flux+update stage kernels and
 the device-side dt/t update) over the whole grid, with the state resident
@@ -100,7 +102,8 @@ def prewarm(advance, sync):
 
 
 WORKLOADS = {"radsod256": (radsod_setup, 256, "radial_sod_3d"), "sphere192": (sphere_setup, 192, "sphere_3d"),
-             "batch512": (None, 512, "batch_2d_mixed"), "learned128": (None, 128, "learned_2d")}
+             "batch512": (None, 512, "batch_2d_mixed"), "learned128": (None, 128, "learned_2d"),
+             "octree128": (None, 128, "octree_sod_3d")}
 MLP_FLOPS_PER_CELL = 2 * (100 * 64 + 64 * 64 + 64 * 16)
 
 
@@ -439,6 +442,135 @@ def run_learned(args, mmf):
     print(json.dumps(out), flush=True)
 
 
+def mesh_bytes(fm):
+    """Algorithmic HBM bytes of one stage of the face-list kernels (axis-aligned
+    faces, a handful of areas: 12 B of geometry per face).  Face kernel: owner,
+    neighbour, kind (12 B) and A*F written (40 B) per face, the stage state read
+    once per cell (40 B).  Cell kernel: offset (4 B) and volume (8 B) per cell,
+    one 4 B incidence per face side, A*F read once per face (40 B), the RK
+    update's state traffic per cell (80 / 120 / 120 B for stages 1 / 2 / 3)."""
+    import numpy as np
+    F, N = fm.nfaces, fm.ncells
+    sides = int(2 * F - (fm.neigh < 0).sum())
+    face = 52.0 * F + 40.0 * N
+    cell = 12.0 * N + 4.0 * sides + 40.0 * F + BYTES_PER_CELL_STAGE * N
+    return face, cell
+
+
+def cpu_baseline_mesh(fm, pb_o, budget_s=10.0, budget_mt=5.0):
+    """The oracle's face-list loop (orc_solver_init_faces) stepping the same
+    mesh from its initial state: about budget_s on one thread, then about
+    budget_mt on all host cores (faces, then per-cell sums in face order)."""
+    from oracle import oracle as O
+    s = O.Solver.from_faces(pb_o, fm.dim, fm.volume, fm.owner, fm.neigh, fm.area, fm.normal, fm.min_cell_size,
+                            border_bc=fm.face_bc, border_info=fm.face_info, centroids=fm.centroid)
+    threads = cpu_threads()
+    res = []
+    for nt, budget in ((1, budget_s), (threads, budget_mt)):
+        U = s.initial_state()
+        W = U.copy()
+        t, k = 0.0, 0
+        t0 = time.perf_counter()
+        while k < 1 or time.perf_counter() - t0 < budget:
+            dt, _ = s.rk3_step(U, W, 0.5, t, 1e30, threads=nt)
+            t += dt
+            k += 1
+        el = time.perf_counter() - t0
+        res.append((fm.ncells * 3 * k / el, "%d SSP-RK3 step(s) of the %d cells in %.1f s" % (k, fm.ncells, el)))
+    return _both(res[0], res[1], threads, "the same octree mesh from its initial state")
+
+
+def run_mesh(args, mmf):
+    """The explicit-interface-list path on a 2:1 refined octree mesh: radial Sod
+    on [-1,1]^3, a 128^3 base grid whose cells inside |x|_inf < 0.5 are split
+    in eight (hanging faces on the refinement boundary), free-flow borders."""
+    from minimmerflow_amd import meshes
+    from oracle import oracle as O
+    n = args.n or 128
+    t0 = time.perf_counter()
+    fm = meshes.refined(3, [n] * 3, [-1.0] * 3, 2.0 / n, ([-0.5] * 3, [0.5] * 3), [0] * 6)
+    pb = mmf.problem("radsod", 3, sod_r0=0.4, domain_origin=[-1.0] * 3, domain_length=2.0)
+    U0 = mmf.exact_conservatives(pb, fm.centroid, 0.0)
+
+    def make():
+        g = mmf.Mesh(fm)
+        g.set_state(U0)
+        g.set_time(0.0, 1e30, 0.5)
+        return g
+
+    g = make()
+    log("setup %.1fs: octree mesh %d cells (%d fine), %d faces" % (time.perf_counter() - t0, fm.ncells,
+                                                                     int((fm.size < 2.0 / n).sum()), fm.nfaces))
+    w = make()
+    nprewarm = prewarm(w.advance, lambda: w.time())
+    w.close()
+    g.advance(args.warmup)
+    g.time()
+    g.enable_timing(True)
+    t1 = time.perf_counter()
+    g.advance(args.steps)
+    t, dt, step, _ = g.time()
+    el = time.perf_counter() - t1
+    face_ms, cell_ms, stages = g.read_timing()
+    g.enable_timing(False)
+    if step != args.warmup + args.steps:
+        raise SystemExit("bench: only %d of %d steps were active" % (step, args.warmup + args.steps))
+    npc = g.counters()
+    if npc[0] or npc[1] or npc[2]:
+        raise SystemExit("bench: negative or non-finite states encountered (%d, %d, %d)" % npc)
+    N = fm.ncells
+    value = N * 3 * args.steps / el
+    fb, cb = mesh_bytes(fm)
+    fa, ca = face_ms / stages, cell_ms / stages
+    dom = "face" if fa >= ca else "cell"
+    achieved = (fb / (fa * 1e-3) if dom == "face" else cb / (ca * 1e-3)) / 1e9
+    workload = "octree_sod_3d_%d_order1" % n
+    log("timed %d steps in %.3f s: %.3f ms/step; face kernel %.4f ms (%.0f GB/s), cell kernel %.4f ms (%.0f GB/s)"
+        % (args.steps, el, el / args.steps * 1e3, fa, fb / fa / 1e6, ca, cb / ca / 1e6))
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle face-list loop ...")
+        cpu = cpu_baseline_mesh(fm, O.problem(O.RADSOD, 3, sod_r0=0.4))
+        log("cpu baseline: %.3e cell-updates/s (single thread %.3e)" % (cpu["value"], cpu["single_thread"]["value"]))
+    out = {
+        "metric": "cell-updates/s per RK stage",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (initial condition sampled at cell centroids)",
+        "config": {"workload": workload, "cells": N, "faces": fm.nfaces,
+                   "mesh": "%d^3 base cells on [-1,1]^3, cells inside |x|_inf < 0.5 split in eight (2:1, hanging "
+                           "faces), explicit interface list" % n,
+                   "initial": "r<0.4: rho=1 p=1; else rho=0.125 p=0.1; u=0", "boundaries": "free flow",
+                   "time": "SSP-RK3, CFL 0.5 (min cell size), fixed step count", "prewarm_steps": nprewarm,
+                   "reconstruction": "order 1", "flux": "Rusanov", "t_after_timed_steps": t,
+                   "kernels_ms_per_stage": {"faces": fa, "cells": ca},
+                   "algorithmic_bytes_per_stage": {"faces": fb, "cells": cb},
+                   "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(workload),
+            "kernel": "k_mesh_%s (%s)" % (dom + "s", "per-interface flux, A*F per face" if dom == "face"
+                                          else "segmented sum in face order + RK update"),
+            "algorithmic_bytes_per_launch": fb if dom == "face" else cb,
+            "avg_launch_ms": fa if dom == "face" else ca,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if os.path.exists(p):
@@ -488,6 +620,9 @@ def main():
         return
     if args.workload == "learned128":
         run_learned(args, mmf)
+        return
+    if args.workload == "octree128":
+        run_mesh(args, mmf)
         return
 
     setup, n_default, wname = WORKLOADS[args.workload]

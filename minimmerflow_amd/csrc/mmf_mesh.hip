@@ -60,7 +60,16 @@ __global__ __launch_bounds__(MB) void k_mesh_faces(MeshDev m, const double *__re
     const bool ns = nb >= 0 && mesh_solved(m, nb);
     if (os || ns) {  // euler.cpp:181-183
       const int32_t kind = m.kind[f];
-      const double n[3] = {m.normal[f], m.normal[m.F + f], m.normal[2 * m.F + f]};
+      const int32_t nc = (kind >> MESH_KIND_NRM_SHIFT) & 7;
+      double n[3];
+      if (nc == MESH_NRM_GENERAL) {
+        n[0] = m.normal[f];
+        n[1] = m.normal[m.F + f];
+        n[2] = m.normal[2 * m.F + f];
+      } else {  // +-e_a: the same doubles as the caller's (1 or -1, and +0)
+        n[0] = n[1] = n[2] = 0.0;
+        n[nc >> 1] = (nc & 1) ? -1.0 : 1.0;
+      }
       double L[NF], R[NF];
       if ((kind & MESH_KIND_BC_MASK) == 0) {
 #pragma unroll
@@ -82,7 +91,8 @@ __global__ __launch_bounds__(MB) void k_mesh_faces(MeshDev m, const double *__re
       }
       double Fx[NF];
       eval_splitting(L, R, n, Fx, &lam, &np, &nr);
-      const double a = m.area[f];
+      const int32_t as = (kind >> MESH_KIND_AREA_SHIFT) & 255;
+      const double a = (as == MESH_AREA_GENERAL) ? m.area[f] : m.area_tab[as];
 #pragma unroll
       for (int k = 0; k < NF; ++k) AF[k * m.F + f] = a * Fx[k];  // euler.cpp:239,245: interfaceArea * fluxes[k]
     }

@@ -28,12 +28,13 @@ struct mmf_mesh {
   hipStream_t stream = nullptr;
   double *dU = nullptr, *dW = nullptr, *dR = nullptr, *dAF = nullptr;
   int32_t *downer = nullptr, *dneigh = nullptr, *dkind = nullptr, *doff = nullptr, *dinc = nullptr;
-  double *darea = nullptr, *dnormal = nullptr, *dinfo = nullptr, *dvol = nullptr;
+  double *darea = nullptr, *dnormal = nullptr, *dinfo = nullptr, *dvol = nullptr, *darea_tab = nullptr;
   uint8_t *dflags = nullptr;
   Ctrl *dctrl = nullptr;
   Ctrl *hctrl = nullptr;
-  // host copies needed to rebuild the face kinds when the cell flags change
-  std::vector<int32_t> owner, neigh, border_kind;
+  // host copies needed to rebuild the face kinds when the cell flags change:
+  // border faces' kinds, every face's geometry bits (normal code, area slot)
+  std::vector<int32_t> owner, neigh, border_kind, geo;
   std::string err;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> fev, cev;
@@ -88,10 +89,10 @@ int upload_kinds(mmf_mesh *m, const std::vector<uint8_t> &fluid) {
   for (int64_t f = 0; f < F; ++f) {
     const int32_t nb = m->neigh[(size_t)f];
     if (nb < 0) {
-      k[(size_t)f] = m->border_kind[(size_t)f];
+      k[(size_t)f] = m->border_kind[(size_t)f] | m->geo[(size_t)f];
     } else {
       const bool wall = fluid.empty() ? false : (fluid[(size_t)m->owner[(size_t)f]] != fluid[(size_t)nb]);
-      k[(size_t)f] = wall ? (MMF_BC_WALL + 1) : 0;
+      k[(size_t)f] = (wall ? (MMF_BC_WALL + 1) : 0) | m->geo[(size_t)f];
     }
   }
   MCHK(m, hipMemcpy(m->dkind, k.data(), sizeof(int32_t) * (size_t)F, hipMemcpyHostToDevice));
@@ -192,7 +193,7 @@ int mmf_mesh_destroy(mmf_mesh *m) {
   drop_graphs(m);  // before the buffers their kernel nodes point at
   for (void *p : {(void *)m->dU, (void *)m->dW, (void *)m->dR, (void *)m->dAF, (void *)m->downer, (void *)m->dneigh,
                   (void *)m->dkind, (void *)m->doff, (void *)m->dinc, (void *)m->darea, (void *)m->dnormal,
-                  (void *)m->dinfo, (void *)m->dvol, (void *)m->dflags, (void *)m->dctrl})
+                  (void *)m->dinfo, (void *)m->dvol, (void *)m->dflags, (void *)m->dctrl, (void *)m->darea_tab})
     if (p) (void)hipFree(p);
   if (m->hctrl) (void)hipHostFree(m->hctrl);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -244,6 +245,41 @@ int mmf_mesh_create(const mmf_mesh_desc *d, mmf_mesh **out) {
     }
     bkind[(size_t)f] = (bc + 1) | (slot << mmf::MESH_KIND_SLOT_SHIFT);
   }
+  if (slots.size() > (size_t)mmf::MESH_MAX_SLOTS) return MMF_ERR_INVALID;
+  // geometry bits: axis-aligned unit normals (exactly +-1 and +0) as a code,
+  // areas from a table of up to 255 distinct values; the rest from arrays
+  std::vector<int32_t> geo((size_t)F);
+  std::vector<double> area_tab;
+  std::map<uint64_t, int32_t> area_slot;
+  bool need_area = false, need_normal = false;
+  for (int64_t f = 0; f < F; ++f) {
+    const double *nv = d->face_normal + 3 * f;
+    int32_t code = mmf::MESH_NRM_GENERAL;
+    for (int a = 0; a < 3 && code == mmf::MESH_NRM_GENERAL; ++a) {
+      bool others_zero = true;
+      for (int b = 0; b < 3; ++b) {
+        uint64_t bits;
+        std::memcpy(&bits, &nv[b], sizeof(bits));
+        if (b != a && bits != 0) others_zero = false;  // +0.0 only
+      }
+      if (others_zero && nv[a] == 1.0) code = 2 * a;
+      else if (others_zero && nv[a] == -1.0) code = 2 * a + 1;
+    }
+    need_normal = need_normal || code == mmf::MESH_NRM_GENERAL;
+    uint64_t abits;
+    std::memcpy(&abits, &d->face_area[f], sizeof(abits));
+    int32_t as = mmf::MESH_AREA_GENERAL;
+    auto it = area_slot.find(abits);
+    if (it != area_slot.end()) {
+      as = it->second;
+    } else if ((int)area_tab.size() < mmf::MESH_MAX_AREAS) {
+      as = (int32_t)area_tab.size();
+      area_slot.emplace(abits, as);
+      area_tab.push_back(d->face_area[f]);
+    }
+    need_area = need_area || as == mmf::MESH_AREA_GENERAL;
+    geo[(size_t)f] = (code << mmf::MESH_KIND_NRM_SHIFT) | (as << mmf::MESH_KIND_AREA_SHIFT);
+  }
   // incidences per cell in increasing face index: the order the interface
   // loop (euler.cpp:153-247) adds them to each residual
   for (int64_t c = 0; c < N; ++c) count[(size_t)c + 1] += count[(size_t)c];
@@ -255,9 +291,12 @@ int mmf_mesh_create(const mmf_mesh_desc *d, mmf_mesh **out) {
       if (neigh[(size_t)f] >= 0) inc[(size_t)pos[(size_t)neigh[(size_t)f]]++] = (int32_t)((f << 1) | 1);
     }
   }
-  std::vector<double> nrm((size_t)(3 * F));
-  for (int64_t f = 0; f < F; ++f)
-    for (int a = 0; a < 3; ++a) nrm[(size_t)(a * F + f)] = d->face_normal[3 * f + a];
+  std::vector<double> nrm;
+  if (need_normal) {
+    nrm.resize((size_t)(3 * F));
+    for (int64_t f = 0; f < F; ++f)
+      for (int a = 0; a < 3; ++a) nrm[(size_t)(a * F + f)] = d->face_normal[3 * f + a];
+  }
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MMF_ERR_NO_DEVICE;
@@ -280,13 +319,16 @@ int mmf_mesh_create(const mmf_mesh_desc *d, mmf_mesh **out) {
       hipHostMalloc(&m->hctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
     return cleanup(MMF_ERR_OUT_OF_MEMORY);
   if (upload(m, &m->downer, owner.data(), (size_t)F) || upload(m, &m->dneigh, neigh.data(), (size_t)F) ||
-      upload(m, &m->darea, d->face_area, (size_t)F) || upload(m, &m->dnormal, nrm.data(), (size_t)(3 * F)) ||
+      (need_area && upload(m, &m->darea, d->face_area, (size_t)F)) ||
+      (need_normal && upload(m, &m->dnormal, nrm.data(), nrm.size())) ||
+      upload(m, &m->darea_tab, area_tab.data(), area_tab.size()) ||
       upload(m, &m->dinfo, info.data(), info.size()) || upload(m, &m->dvol, d->cell_volume, (size_t)N) ||
       upload(m, &m->doff, count.data(), (size_t)N + 1) || upload(m, &m->dinc, inc.data(), inc.size()))
     return cleanup(MMF_ERR_OUT_OF_MEMORY);
   m->owner.swap(owner);
   m->neigh.swap(neigh);
   m->border_kind.swap(bkind);
+  m->geo.swap(geo);
   if (upload_kinds(m, {}) != MMF_OK) return cleanup(MMF_ERR_HIP);
   if (hipMemset(m->dU, 0, sb) != hipSuccess || hipMemset(m->dW, 0, sb) != hipSuccess ||
       hipMemset(m->dR, 0, sb) != hipSuccess || hipMemset(m->dctrl, 0, sizeof(Ctrl)) != hipSuccess ||
@@ -297,6 +339,7 @@ int mmf_mesh_create(const mmf_mesh_desc *d, mmf_mesh **out) {
   m->m.neigh = m->dneigh;
   m->m.area = m->darea;
   m->m.normal = m->dnormal;
+  m->m.area_tab = m->darea_tab;
   m->m.kind = m->dkind;
   m->m.info = m->dinfo;
   m->m.vol = m->dvol;

@@ -714,11 +714,12 @@ void orc_reconstruct(const orc_solver *s, int64_t cell, const double *outward,
         memcpy(values, mean, sizeof(double) * ORC_NF);
 }
 
-/* One interface of euler.cpp:153-247 (the body of the face loop): fluxes of
- * face f accumulated into the residual of its solved sides, its eigenvalue
- * into *max_eig, negative p / rho sides counted. */
-static void orc_face(const orc_solver *s, const double *U, double *RHS, int64_t f, double *max_eig,
-                     int64_t *neg_p, int64_t *neg_rho)
+/* One interface of euler.cpp:153-247 (the body of the face loop) up to the
+ * sums: AF = interfaceArea * fluxes, the eigenvalue into *max_eig, negative
+ * p / rho sides counted.  Returns which sides take AF: bit 0 the owner
+ * (RHS -= AF), bit 1 the neighbour (RHS += AF); 0 for a skipped face. */
+static int orc_face_af(const orc_solver *s, const double *U, int64_t f, double *AF, double *max_eig,
+                       int64_t *neg_p, int64_t *neg_rho)
 {
     const orc_mesh *m = &s->mesh;
     int64_t owner = m->face_owner[f];
@@ -727,15 +728,15 @@ static void orc_face(const orc_solver *s, const double *U, double *RHS, int64_t 
     int neighSolved = (neigh >= 0) ? s->solved[neigh] : 0;
 
 #ifndef ORC_PLANT_NO_SKIP /* planted mistake (tests only): faces of unsolved cells evaluated */
-    if (!ownerSolved && !neighSolved) return;
+    if (!ownerSolved && !neighSolved) return 0;
 #else
     if (neigh >= 0 && s->face_bc[f] == ORC_BC_NONE && !ownerSolved && !neighSolved) {
         double Fs[ORC_NF], es;
         orc_eval_splitting(&U[ORC_NF * owner], &U[ORC_NF * neigh], &m->face_normal[3 * f], Fs, &es, NULL, NULL);
         *max_eig = (es < *max_eig) ? *max_eig : es;
-        return;
+        return 0;
     }
-    if (!ownerSolved && !neighSolved) return;
+    if (!ownerSolved && !neighSolved) return 0;
 #endif
 
     int bc = s->face_bc[f];
@@ -786,10 +787,21 @@ static void orc_face(const orc_solver *s, const double *U, double *RHS, int64_t 
     ownerSolved = 1;
     neighSolved = neigh >= 0;
 #endif
-    if (ownerSolved)
-        for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * owner + k] -= area * F[k];
-    if (neighSolved)
-        for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * neigh + k] += area * F[k];
+    for (int k = 0; k < ORC_NF; ++k) AF[k] = area * F[k];
+    return (ownerSolved ? 1 : 0) | (neighSolved ? 2 : 0);
+}
+
+/* One interface with its sums: RHS_owner -= AF, RHS_neigh += AF
+ * (euler.cpp:237-247). */
+static void orc_face(const orc_solver *s, const double *U, double *RHS, int64_t f, double *max_eig,
+                     int64_t *neg_p, int64_t *neg_rho)
+{
+    double AF[ORC_NF];
+    const int sides = orc_face_af(s, U, f, AF, max_eig, neg_p, neg_rho);
+    if (sides & 1)
+        for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * s->mesh.face_owner[f] + k] -= AF[k];
+    if (sides & 2)
+        for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * s->mesh.face_neigh[f] + k] += AF[k];
 }
 
 /* euler.cpp:127-249 */
@@ -877,6 +889,7 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
 
     orc_polys(s, U, nthreads);
     if (nthreads > 1 && !m->face_area) orc_compute_rhs_threads(s, U, RHS, &maxEig, nthreads);
+    else if (nthreads > 1) orc_compute_rhs_faces_threads(s, U, RHS, &maxEig, nthreads);
     else orc_compute_rhs(s, U, RHS, &maxEig);
     if (stage_eig) stage_eig[0] = maxEig;
 
@@ -898,6 +911,7 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
 
     orc_polys(s, W, nthreads);
     if (nthreads > 1 && !m->face_area) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
+    else if (nthreads > 1) orc_compute_rhs_faces_threads(s, W, RHS, &maxEig, nthreads);
     else orc_compute_rhs(s, W, RHS, &maxEig);
     if (stage_eig) stage_eig[1] = maxEig;
 
@@ -916,6 +930,7 @@ static double orc_rk3(orc_solver *s, double *U, double *W, double *RHS, double c
 
     orc_polys(s, W, nthreads);
     if (nthreads > 1 && !m->face_area) orc_compute_rhs_threads(s, W, RHS, &maxEig, nthreads);
+    else if (nthreads > 1) orc_compute_rhs_faces_threads(s, W, RHS, &maxEig, nthreads);
     else orc_compute_rhs(s, W, RHS, &maxEig);
     if (stage_eig) stage_eig[2] = maxEig;
 
@@ -1017,6 +1032,72 @@ void orc_solver_flags(const orc_solver *s, uint8_t *fluid, uint8_t *solved)
 void orc_solver_face_bc(const orc_solver *s, int *out)
 {
     memcpy(out, s->face_bc, sizeof(int) * s->mesh.nfaces);
+}
+
+/* The residual of a face-list solver on nthreads threads, bit for bit the
+ * serial interface loop: every face's A*F and eigenvalue first (faces in
+ * parallel), then every cell sums its faces in increasing face index (cells
+ * in parallel), the order the serial loop adds them. */
+int orc_compute_rhs_faces_threads(orc_solver *s, const double *U, double *RHS, double *max_eig, int nthreads)
+{
+    const orc_mesh *m = &s->mesh;
+    const int64_t F = m->nfaces, N = m->ncells;
+    double *AF = (double *)malloc(sizeof(double) * ORC_NF * F);
+    uint8_t *live = (uint8_t *)malloc(F);
+    int64_t *off = (int64_t *)calloc((size_t)N + 1, sizeof(int64_t));
+    int64_t *inc = (int64_t *)malloc(sizeof(int64_t) * 2 * F);
+    if (!AF || !live || !off || !inc) {
+        free(AF); free(live); free(off); free(inc);
+        return -1;
+    }
+    for (int64_t f = 0; f < F; ++f) {
+        off[m->face_owner[f] + 1] += 1;
+        if (m->face_neigh[f] >= 0) off[m->face_neigh[f] + 1] += 1;
+    }
+    for (int64_t c = 0; c < N; ++c) off[c + 1] += off[c];
+    {
+        int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * N);
+        if (!pos) { free(AF); free(live); free(off); free(inc); return -1; }
+        memcpy(pos, off, sizeof(int64_t) * N);
+        for (int64_t f = 0; f < F; ++f) {
+            inc[pos[m->face_owner[f]]++] = 2 * f;
+            if (m->face_neigh[f] >= 0) inc[pos[m->face_neigh[f]]++] = 2 * f + 1;
+        }
+        free(pos);
+    }
+    double gmax = 0.0;
+    int64_t gp = 0, gr = 0;
+#pragma omp parallel num_threads(nthreads) if (nthreads > 1)
+    {
+        double lmax = 0.0;
+        int64_t lp = 0, lr = 0;
+#pragma omp for schedule(static)
+        for (int64_t f = 0; f < F; ++f)
+            live[f] = (uint8_t)orc_face_af(s, U, f, &AF[ORC_NF * f], &lmax, &lp, &lr);
+#pragma omp for schedule(static)
+        for (int64_t c = 0; c < N; ++c) {
+            for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * c + k] = 0.;
+            for (int64_t e = off[c]; e < off[c + 1]; ++e) {
+                const int64_t f = inc[e] >> 1;
+                if (!(live[f] & ((inc[e] & 1) ? 2 : 1))) continue;
+                if (inc[e] & 1)
+                    for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * c + k] += AF[ORC_NF * f + k];
+                else
+                    for (int k = 0; k < ORC_NF; ++k) RHS[ORC_NF * c + k] -= AF[ORC_NF * f + k];
+            }
+        }
+#pragma omp critical
+        {
+            gmax = (lmax < gmax) ? gmax : lmax;
+            gp += lp;
+            gr += lr;
+        }
+    }
+    s->neg_p += gp;
+    s->neg_rho += gr;
+    *max_eig = gmax;
+    free(AF); free(live); free(off); free(inc);
+    return 0;
 }
 
 /* The solver's face list (owner, neighbour, owner-outward normal, area) and

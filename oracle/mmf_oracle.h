@@ -170,6 +170,10 @@ double orc_rk3_step(orc_solver *s, double *U, double *work, double *RHS,
 int orc_compute_rhs_threads(orc_solver *s, const double *U, double *RHS, double *max_eig, int nthreads);
 double orc_rk3_step_threads(orc_solver *s, double *U, double *work, double *RHS,
                             double cfl, double t, double tmax, double *stage_eig, int nthreads);
+/* Face-list solvers (orc_solver_init_faces): every face's A*F in parallel,
+ * then every cell's sum in increasing face index in parallel (the serial
+ * loop's order).  orc_rk3_step_threads uses it for them. */
+int orc_compute_rhs_faces_threads(orc_solver *s, const double *U, double *RHS, double *max_eig, int nthreads);
 /* Loop until t >= tmax; returns number of steps, final t in *t_out. */
 int64_t orc_run(orc_solver *s, double *U, double cfl, double t0, double tmax,
                 int64_t max_steps, double *t_out);

@@ -115,6 +115,8 @@ def _load(path):
     L.orc_rk3_step_threads.restype = C.c_double
     L.orc_compute_rhs_threads.argtypes = [C.c_void_p, dp, dp, dp, C.c_int]
     L.orc_compute_rhs_threads.restype = C.c_int
+    L.orc_compute_rhs_faces_threads.argtypes = [C.c_void_p, dp, dp, dp, C.c_int]
+    L.orc_compute_rhs_faces_threads.restype = C.c_int
     L.orc_run.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int64, dp]
     L.orc_run.restype = C.c_int64
     L.orc_error.argtypes = [C.c_void_p, dp, C.c_double, dp, dp]
@@ -316,7 +318,9 @@ class Solver:
         R = np.empty((self.ncells, NF))
         me = C.c_double()
         lib().orc_compute_polynomials(self.s, _dp(U))
-        if threads > 1:
+        if threads > 1 and getattr(self, "_keep", None) is not None:
+            lib().orc_compute_rhs_faces_threads(self.s, _dp(U), _dp(R), C.byref(me), int(threads))
+        elif threads > 1:
             lib().orc_compute_rhs_threads(self.s, _dp(U), _dp(R), C.byref(me), int(threads))
         else:
             lib().orc_compute_rhs(self.s, _dp(U), _dp(R), C.byref(me))

@@ -36,7 +36,26 @@ def mesh_cases():
                                       [0, 1, 3, 0], INFO), 0.05),
         ("3d refined", meshes.refined(3, [10, 9, 8], [0.0] * 3, 0.5, ([1.0, 1.0, 1.0], [3.0, 2.5, 2.6]),
                                       [0, 1, 3, 0, 2, 0], INFO), 0.05),
+        ("2d general normals and areas", general_geometry(
+            meshes.cartesian(2, [30, 21], [0.0] * 3, 0.25, [3, 1, 2, 0], INFO), 1), 0.1),
+        ("3d general normals and areas", general_geometry(
+            meshes.refined(3, [8, 7, 6], [0.0] * 3, 0.5, ([1.0, 1.0, 1.0], [2.5, 2.0, 2.0]), [1, 3, 0, 2, 0, 1],
+                           INFO), 2), 0.05),
     ]
+
+
+def general_geometry(fm, seed):
+    """The same connectivity with every normal rotated off the axes and areas
+    jittered face by face: the normal and area arrays are read for every face
+    (no axis code, more than 255 distinct areas).  Parity only: the cells no
+    longer close."""
+    rng = np.random.default_rng(seed)
+    c, s_ = np.cos(0.5), np.sin(0.5)
+    rot = np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+    nrm = fm.normal @ rot.T
+    area = fm.area * rng.uniform(0.9, 1.1, fm.nfaces)
+    return meshes.FaceMesh(fm.dim, fm.volume, fm.size, fm.centroid, fm.owner, fm.neigh, area, nrm, fm.face_centroid,
+                           fm.face_bc, fm.face_info)
 
 
 def fluid_mask(rng, fm, frac):

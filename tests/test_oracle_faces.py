@@ -164,3 +164,29 @@ def test_refined_vortex_is_more_accurate_than_the_coarse_grid(orc):
         s.run(U, 0.45, 0.0, 0.5)
         errs[key] = s.error(U, 0.5)[0]
     assert errs["fine"] < errs["refined"] < errs["coarse"], errs
+
+
+def test_threaded_face_list_steps_equal_serial(orc):
+    """The all-cores form of the face-list loop (faces, then per-cell sums in
+    face order) gives the serial loop's residual and steps bit for bit."""
+    fm = meshes.refined(3, [9, 8, 7], [0.0] * 3, 0.5, ([1.0, 1.0, 1.0], [3.0, 2.5, 2.0]), [3, 1, 0, 2, 0, 1], INFO)
+    fm = fm.permuted(meshes.morton_order(fm))
+    pb = orc.problem(orc.RADSOD, 3)
+    rng = np.random.default_rng(11)
+    fluid = (rng.uniform(size=fm.ncells) > 0.1).astype(np.uint8)
+    U0 = np.array([orc.prim2cons([rng.uniform(0.5, 2), *rng.uniform(-0.6, 0.6, 3), rng.uniform(0.5, 2)])
+                   for _ in range(fm.ncells)])
+    out = {}
+    for threads in (1, 4):
+        s = face_solver(orc, pb, fm, fluid)
+        R, me = s.compute_rhs(U0, threads=threads)
+        U, W = U0.copy(), U0.copy()
+        t, steps = 0.0, []
+        for _ in range(2):
+            dt, eig = s.rk3_step(U, W, 0.5, t, 1e30, threads=threads)
+            t += dt
+            steps.append((dt, tuple(eig)))
+        out[threads] = (R, me, s.counters(), steps, U)
+    a, b = out[1], out[4]
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2] and a[3] == b[3]
+    assert np.array_equal(a[4], b[4])
