@@ -590,7 +590,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="cells per axis (default: the workload's)")
     ap.add_argument("--order", type=int, default=1, choices=(1, 2))
     ap.add_argument("--kernel", choices=("auto", "gather", "march", "tile"), default="auto")
-    ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4 or 8)")
+    ap.add_argument("--rows", type=int, default=0, help="marching kernel: tile rows (4, 8 or 16)")
+    ap.add_argument("--cols", type=int, default=0, help="marching kernel: tile columns (64 or 32)")
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
     ap.add_argument("--ncases", type=int, default=512, help="batch: number of cases")
@@ -633,6 +634,8 @@ def main():
     kind = {"auto": mmf.KERNEL_AUTO, "gather": mmf.KERNEL_GATHER, "march": mmf.KERNEL_MARCH,
             "tile": mmf.KERNEL_TILE}[args.kernel]
     s.set_kernel(kind, args.rows, args.chunk)
+    if args.cols:
+        s.set_march_shape(args.cols, args.rows, args.chunk)
     kinfo = s.kernel()
     solved = N if mask_bytes == 0 else int(((s.cell_flags() & mmf.CELL_SOLVED) != 0).sum())
     log("setup %.1fs: %d^3 cells (%d solved), %d interfaces, order %d"
@@ -640,6 +643,8 @@ def main():
 
     w, _, _ = setup(mmf, n, args.order)  # the pre-warm instance
     w.set_kernel(kind, args.rows, args.chunk)
+    if args.cols:
+        w.set_march_shape(args.cols, args.rows, args.chunk)
     cfg["prewarm_steps"] = prewarm(w.advance, w.synchronize)
     s.advance(args.warmup)
     s.synchronize()
@@ -701,7 +706,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": load_traffic(workload),
             "kernel": "%s (fused flux + RK stage), mean of stages 1-3" % (
-                "k_march tile 64x%d, %d planes/block" % (kinfo[1], kinfo[2]) if kinfo[0] == mmf.KERNEL_MARCH
+                "k_march tile %dx%d, %d planes/block" % (args.cols or 64, kinfo[1], kinfo[2])
+                if kinfo[0] == mmf.KERNEL_MARCH
                 else "k_stage (per-cell gather)"),
             "algorithmic_bytes_per_launch": bytes_per_cell * N,
             "avg_launch_ms": avg_ms,

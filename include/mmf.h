@@ -97,6 +97,10 @@ enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2, MMF_KER
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk);
 /* kind actually used for the next launches, and the marching shape */
 int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk);
+/* The 3D marching kernel's tile (tile_x x tile_y cells: 64 x 4, 64 x 8,
+ * 32 x 8 or 32 x 16) and chunk; <= 0 keeps the default of that field.  Does
+ * not change the kernel kind.  Results do not depend on the shape. */
+int mmf_set_march_shape(mmf_ctx *ctx, int32_t tile_x, int32_t tile_y, int32_t chunk);
 
 /* ---- body / flags (body::isPointFluid + main.cpp:222-237, 251-277) ------- */
 /* fluid: [ncells] 0/1 or NULL (all fluid); solved: NULL -> solved = fluid */

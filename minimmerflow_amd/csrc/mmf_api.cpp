@@ -300,9 +300,25 @@ int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk)
   mmf::MarchShape sh = mmf::default_march_shape(ctx->g, ctx->num_cus);
   if (tile_rows > 0) sh.ty = tile_rows;
   if (chunk > 0) sh.chunk = chunk;
+  if (tile_rows > 0 && sh.tx == 64 && tile_rows == 16 && ctx->g.dim == 3) sh.tx = 32;  // 64 x 16 does not fit; 32 x 16
   if (kind == mmf::KERNEL_MARCH && !mmf::march_shape_supported(ctx->g, sh))
     return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1, or order 2 in 2D without bodies; fewer than 2^29 cells; tile rows 4 or 8 in 3D, 1 in 2D");
   ctx->kernel = kind;
+  ctx->shape = sh;
+  return MMF_OK;
+}
+
+int mmf_set_march_shape(mmf_ctx *ctx, int32_t tile_x, int32_t tile_y, int32_t chunk) {
+  if (!ctx) return MMF_ERR_INVALID;
+  mmf::MarchShape sh = ctx->shape;
+  const mmf::MarchShape def = mmf::default_march_shape(ctx->g, ctx->num_cus);
+  sh.tx = tile_x > 0 ? tile_x : def.tx;
+  sh.ty = tile_y > 0 ? tile_y : def.ty;
+  sh.chunk = chunk > 0 ? chunk : def.chunk;
+  if (!mmf::march_shape_supported(ctx->g, sh))
+    return fail(ctx, MMF_ERR_INVALID, "marching shape: 3D tiles 64 x 4, 64 x 8, 32 x 8 or 32 x 16; 2D 64 x 1");
+  ctx->lam_valid = false;
+  ctx->tiles_valid = false;
   ctx->shape = sh;
   return MMF_OK;
 }

@@ -46,8 +46,9 @@ enum KernelKind { KERNEL_AUTO = 0, KERNEL_GATHER = 1, KERNEL_MARCH = 2, KERNEL_T
 
 // Launch shape of the marching kernel (tile rows and march chunk).
 struct MarchShape {
-  int ty;     // tile rows (3D): 4, 8 or 16; 1 in 2D
-  int chunk;  // planes per block along the march axis
+  int ty;       // tile rows (3D): 4, 8 or 16; 1 in 2D
+  int chunk;    // planes per block along the march axis
+  int tx = 64;  // tile columns (3D): 64 (one wave per row) or 32
 };
 
 // Stage maxima and block counters are spread over EIG_SPLIT cache lines each

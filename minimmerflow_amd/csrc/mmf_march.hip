@@ -36,7 +36,7 @@ namespace mmf {
 
 namespace {
 
-constexpr int TX = 64;
+constexpr int TX2D = 64;  // the 2D strip kernels' wave width (default_march_shape)
 constexpr int NTUP = 11;  // c[5], vel[3], p, a, H
 constexpr int64_t MARCH_MAX_CELLS = (int64_t)1 << 29;
 
@@ -186,13 +186,32 @@ __device__ __forceinline__ double virtual_face(const GridDev &g, const double *b
   return lam;
 }
 
+// The largest of a cell's per-axis eigenvalues |u_a| + sqrt(gamma T)
+// (euler.cpp:57-66 for n = +-e_a) with four divisions instead of the tuple's
+// seven, the same double: a correctly rounded division is odd in its
+// divisor and non-decreasing in its dividend, so max_a |fl(m_a / rho)| =
+// fl(max_a |m_a| / |rho|), and fl(x + a) is non-decreasing in x; T is
+// cons2prim's (utils.cpp:48-63) to the bit.  A NaN momentum wins the u64
+// maximum and stays NaN.
+template <int DIM>
+__device__ __forceinline__ double cell_max_eig(const double *c) {
+  double m = fabs(c[FID_RHO_U]);
+#pragma unroll
+  for (int a = 1; a < DIM; ++a) m = lam_acc(m, fabs(c[FID_RHO_U + a]));
+  const double umax = m / fabs(c[FID_RHO]);
+  const double K = (c[FID_RHO_U] * c[FID_RHO_U] + c[FID_RHO_V] * c[FID_RHO_V] + c[FID_RHO_W] * c[FID_RHO_W]) /
+                   (c[FID_RHO] * c[FID_RHO]);
+  const double T = (2.0 * c[FID_RHO_E] / c[FID_RHO] - K) / (2.0 / (GAMMA - 1.0));
+  return umax + sqrt(GAMMA * T);
+}
+
 }  // namespace
 
 // One block's march over tile 'tile' (x, y tile and chunk).  MASK: the cell
 // flags are read (bodies).  EARLY is the launch's data flow (early dt), so a
 // block of a masked launch that runs the unmasked code keeps it.  lds, lfl
 // and bct are the kernel's LDS arrays.
-template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool EARLY>
+template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool EARLY>
 __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in, const double *U, double *out,
                                             Ctrl *ctrl, int mode, int chunk, int ntx, int nty, int tile,
                                             double *lds, uint8_t *lfl, double *bct) {
@@ -522,15 +541,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
           nu[k] = w1 ? Tc.c[k] + div_vol(gp, dt * rhs) : rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt);
           outf.put(k, idx, nu[k]);
         }
-        if (EARLY && STAGE == 3) {  // the next step's stage-1 eigenvalues of this cell
-          Tup Tq;
-          make_tup(nu, Tq);
-#pragma unroll
-          for (int a = 0; a < DIM; ++a) {
-            const double l = fabs(Tq.vel[a]) + Tq.a;
-            lam_nx = lam_acc(lam_nx, l);
-          }
-        }
+        if (EARLY && STAGE == 3) lam_nx = lam_acc(lam_nx, cell_max_eig<DIM>(nu));
       } else if (STAGE != 3) {
 #pragma unroll
         for (int k = 0; k < NF; ++k)
@@ -568,11 +579,16 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   if (STAGE == 3 && mode == MODE_STEP) close_step<EARLY>(ctrl, tid);
 }
 
-template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
-__global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
-                                                  Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
+// TX x TY tiles: 64 x 4 / 64 x 8 (one wave per row) and 32 x 8 (four waves,
+// 60 KB of LDS: two independent blocks per CU, each SIMD holding a wave of
+// each, so one block's plane barrier and division chains overlap the other's
+// work) or 32 x 16 (eight waves, a halo ring of 19% of the tile's cells
+// instead of 28% for 64 x 8).
+template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL>
+__global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
+                                                     Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
   constexpr int PL = (TY + 2) * (TX + 2);
-  static_assert(TY <= 8, "two tuple planes must fit in LDS");
+  static_assert(2 * NTUP * PL * 8 <= 120 * 1024, "two tuple planes must fit in LDS");
   __shared__ double lds[2 * NTUP * PL];
   __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
   __shared__ double bct[6 * NTUP];
@@ -582,19 +598,19 @@ __global__ __launch_bounds__(TX *TY) void k_march(GridDev g, StageIn<STAGE> in, 
     // neighbour planes are all fluid and solved runs the unmasked code
     const int t = g.march_tiles[blockIdx.x];
     if (t & MARCH_TILE_MASKED)
-      march_block<DIM, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
-                                                         t & ~MARCH_TILE_MASKED, lds, lfl, bct);
+      march_block<DIM, TX, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
+                                                             t & ~MARCH_TILE_MASKED, lds, lfl, bct);
     else
-      march_block<DIM, TY, STAGE, false, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty, t, lds, lfl,
-                                                          bct);
+      march_block<DIM, TX, TY, STAGE, false, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty, t, lds,
+                                                              lfl, bct);
   } else {
-    march_block<DIM, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
+    march_block<DIM, TX, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
                                                        march_tile_of_block((int)blockIdx.x, (int)gridDim.x), lds,
                                                        lfl, bct);
   }
 }
 
-template <int DIM, int TY, int STAGE, bool MASK, bool P2, bool REFL>
+template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL>
 static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mode, const double *U, const double *R,
                                  const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
   StageIn<STAGE> in;
@@ -613,23 +629,23 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
   dim3 grid((unsigned)(ntx * nty * nch));
   dim3 block(TX, TY);
-  hipLaunchKernelGGL((k_march<DIM, TY, STAGE, MASK, P2, REFL>), grid, block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx,
-                     nty);
+  hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL>), grid, block, 0, s, g, in, U, out, ctrl, mode,
+                     sh.chunk, ntx, nty);
   return hipGetLastError();
 }
 
-template <int DIM, int TY, bool MASK, bool P2, bool REFL>
+template <int DIM, int TX, int TY, bool MASK, bool P2, bool REFL>
 static hipError_t launch_march_p(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
   switch (stage) {
-    case 1: return launch_march_t<DIM, TY, 1, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    case 2: return launch_march_t<DIM, TY, 2, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    default: return launch_march_t<DIM, TY, 3, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 1: return launch_march_t<DIM, TX, TY, 1, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return launch_march_t<DIM, TX, TY, 2, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return launch_march_t<DIM, TX, TY, 3, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
 }
 
-template <int DIM, int TY, bool MASK>
+template <int DIM, int TX, int TY, bool MASK>
 static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
@@ -638,23 +654,27 @@ static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int sta
   for (int side = 0; side < 2 * DIM; ++side)
     refl = refl || ((g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) && !g.periodic[side / 2]);
   if (refl) {
-    if (g.pow2A && g.pow2V) return launch_march_p<DIM, TY, MASK, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-    return launch_march_p<DIM, TY, MASK, false, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    if (g.pow2A && g.pow2V)
+      return launch_march_p<DIM, TX, TY, MASK, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return launch_march_p<DIM, TX, TY, MASK, false, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
   if constexpr (!MASK) {
-    if (g.pow2A && g.pow2V) return launch_march_p<DIM, TY, false, true, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-    return launch_march_p<DIM, TY, false, false, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    if (g.pow2A && g.pow2V)
+      return launch_march_p<DIM, TX, TY, false, true, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return launch_march_p<DIM, TX, TY, false, false, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
   return hipErrorInvalidValue;  // not reached (MASK implies refl)
 }
 
 int64_t march_blocks(const GridDev &g, const MarchShape &sh) {
+  const int TX = sh.tx;
   const int64_t ntx = (g.n[0] + TX - 1) / TX, nty = (g.n[1] + sh.ty - 1) / sh.ty;
   return ntx * nty * ((g.n[2] + sh.chunk - 1) / sh.chunk);
 }
 
 void build_march_tiles(const GridDev &g, const MarchShape &sh, const uint8_t *flags, int32_t *tiles) {
   const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int TX = sh.tx;
   const int64_t ntx = (nx + TX - 1) / TX, nty = (ny + sh.ty - 1) / sh.ty;
   const int nblk = (int)march_blocks(g, sh);
   const uint8_t all = CELL_FLUID | CELL_SOLVED;
@@ -710,7 +730,7 @@ bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
   // order 2: the 2D strip kernel, grids without bodies
   if (g.order != 1 && !(g.order == 2 && g.dim == 2 && g.flags == nullptr)) return false;
   if (g.dim == 2) return sh.ty == 1;
-  return sh.ty == 4 || sh.ty == 8;
+  return (sh.tx == 64 && (sh.ty == 4 || sh.ty == 8)) || (sh.tx == 32 && (sh.ty == 8 || sh.ty == 16));
 }
 
 bool march_early(const GridDev &g) {
@@ -724,10 +744,11 @@ bool march_early(const GridDev &g) {
 // block: one 64x8-tile block per CU is resident at a time (LDS and VGPRs).
 MarchShape default_march_shape(const GridDev &g, int num_cus) {
   MarchShape sh;
+  sh.tx = TX2D;
   sh.ty = (g.dim == 2) ? 1 : 8;
   const int M = g.dim - 1;
   // 2D strips are one wave, 62 interior cells wide; several fit on a CU
-  const int64_t ntx = (g.dim == 2) ? (g.n[0] + 61) / 62 : (g.n[0] + TX - 1) / TX;
+  const int64_t ntx = (g.dim == 2) ? (g.n[0] + 61) / 62 : (g.n[0] + sh.tx - 1) / sh.tx;
   const int64_t nty = (g.dim == 3) ? (g.n[1] + sh.ty - 1) / sh.ty : 1;
   const int64_t want = ((num_cus > 0) ? num_cus : 256) * ((g.dim == 2) ? 4 : 1);
   sh.chunk = 128;
@@ -743,13 +764,20 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &sh, int stage,
     if (g.order == 2) return launch_stage_march2d_o2(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
     return launch_stage_march2d(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
+  if (sh.tx == 32) {
+    if (sh.ty == 16)
+      return mask ? launch_march_s<3, 32, 16, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                  : launch_march_s<3, 32, 16, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return mask ? launch_march_s<3, 32, 8, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                : launch_march_s<3, 32, 8, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  }
   switch (sh.ty) {
     case 4:
-      return mask ? launch_march_s<3, 4, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
-                  : launch_march_s<3, 4, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+      return mask ? launch_march_s<3, 64, 4, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                  : launch_march_s<3, 64, 4, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
     default:
-      return mask ? launch_march_s<3, 8, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
-                  : launch_march_s<3, 8, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+      return mask ? launch_march_s<3, 64, 8, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                  : launch_march_s<3, 64, 8, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
 }
 

@@ -12,6 +12,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 32), (4, 7), (8, 128), (8, 1)]
+# 32-wide tiles (cols, rows, chunk)
+SHAPES32 = [(32, 8, 32), (32, 8, 5), (32, 16, 128), (32, 16, 3)]
 
 
 def build(mmf, dim, n, side_bc, info=None, body=None, order=1, seed=0):
@@ -32,8 +34,10 @@ def build(mmf, dim, n, side_bc, info=None, body=None, order=1, seed=0):
     return s, U
 
 
-def run_variant(mmf, s, U, kind, rows=0, chunk=0, steps=3):
+def run_variant(mmf, s, U, kind, rows=0, chunk=0, steps=3, cols=0):
     s.set_kernel(kind, rows, chunk)
+    if cols:
+        s.set_march_shape(cols, rows, chunk)
     s.set_state(U)
     R, me = s.compute_rhs()
     s.set_time(0.0, 10.0, 0.5)
@@ -65,9 +69,9 @@ def test_march_equals_gather(mmf, orc, name, dim, n, bc, body):
     info[:] = [1.0, 0.3, -0.1, 0.05 if dim == 3 else 0.0, 1.0]
     s, U = build(mmf, dim, list(n), bc, info, body)
     ref = run_variant(mmf, s, U, mmf.KERNEL_GATHER)
-    shapes = SHAPES if dim == 3 else [(1, 16), (1, 5), (1, 1000)]
-    for rows, chunk in shapes:
-        got = run_variant(mmf, s, U, mmf.KERNEL_MARCH, rows, chunk)
+    shapes = [(64, r, c) for r, c in SHAPES] + SHAPES32 if dim == 3 else [(0, 1, 16), (0, 1, 5), (0, 1, 1000)]
+    for cols, rows, chunk in shapes:
+        got = run_variant(mmf, s, U, mmf.KERNEL_MARCH, rows, chunk, cols=cols)
         assert s.kernel()[0] == mmf.KERNEL_MARCH
         assert np.array_equal(got[0], ref[0]), (name, rows, chunk, "rhs")
         assert got[1] == ref[1], (name, rows, chunk, "maxEig")
