@@ -360,26 +360,28 @@ def run_learned(args, mmf):
     import numpy as np
     from minimmerflow_amd import learned as LR
     n = args.n or 128
+    theta = LR.THETA if args.theta is None else args.theta
     t0 = time.perf_counter()
     train_cases = LR.case_params(args.train_cases, seed=11)
-    per_case = 40 * 64 * 8
+    per_case = args.snapshots * 64 * 8
     net = mmf.Learner(seed=5, capacity=len(train_cases) * per_case)
-    LR.build_training_set(net, train_cases, 4 * n, 4, 40, 64, seed=1)
+    train_theta = LR.THETA_TRAIN if args.train_theta is None else args.train_theta
+    LR.build_training_set(net, train_cases, 4 * n, 4, args.snapshots, 64, seed=1, theta=train_theta)
     t_data = time.perf_counter() - t0
     t1 = time.perf_counter()
     loss = None
     for r in range(3):
         loss = net.train(3000, batch=8192, lr0=2e-3 if r == 0 else 5e-4, lr1=2e-5, seed=r + 1)
     t_train = time.perf_counter() - t1
-    log("training set %d rows from %d fine %d^2 runs (%.1fs); 9000 Adam steps, batch 8192 (%.1fs), loss %.3e"
-        % (net.num_samples(), len(train_cases), 4 * n, t_data, t_train, loss))
+    log("training set %d rows from %d fine %d^2 runs x %d snapshots (%.1fs); 9000 Adam steps, batch 8192 (%.1fs), "
+        "loss %.3e" % (net.num_samples(), len(train_cases), 4 * n, args.snapshots, t_data, t_train, loss))
     cases = []
     ratios, speed256, speed512 = [], [], []
     for c in LR.case_params(args.eval_cases, seed=99):
         runs = {}
         for m in (n, 2 * n, 4 * n):
             runs["plain%d" % m] = LR.run_case(c, m)
-        runs["learned%d" % n] = LR.run_case(c, n, net=net)
+        runs["learned%d" % n] = LR.run_case(c, n, net=net, theta=theta)
         ref = runs["plain%d" % (4 * n)][0]
         out = {"case": c}
         for key, (sv, pb, steps, wall) in runs.items():
@@ -400,7 +402,7 @@ def run_learned(args, mmf):
     # stage timing of the learned tile kernel (HIP events)
     c = LR.case_params(1, seed=99)[0]
     sv, _ = LR.solver(c, n)
-    sv.set_learned(net, LR.THETA)
+    sv.set_learned(net, theta)
     sv.set_time(0.0, 1e30, LR.CFL)
     sv.advance(3)
     sv.enable_timing(True)
@@ -427,6 +429,8 @@ def run_learned(args, mmf):
         "dtype": "f64 (network bf16 in, fp32 accumulate)",
         "data": "synthetic (seeded initial conditions); training rows built on the device from fine plain runs",
         "config": {"workload": "learned_2d_%d" % n, "train_cases": len(train_cases), "fine": 4 * n,
+                   "snapshots": len(train_cases) * args.snapshots, "sensor_threshold": theta,
+                   "training_sensor_threshold": train_theta if train_theta != float("inf") else None,
                    "training_rows": net.num_samples(), "training_s": t_train, "data_s": t_data, "final_loss": loss,
                    "cases": cases,
                    "wall_speedup_vs_plain_2n": float(np.exp(np.mean(np.log(speed256)))),
@@ -595,8 +599,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
     ap.add_argument("--ncases", type=int, default=512, help="batch: number of cases")
-    ap.add_argument("--train-cases", type=int, default=32, help="learned: fine training runs")
+    ap.add_argument("--train-cases", type=int, default=128, help="learned: fine training runs")
     ap.add_argument("--eval-cases", type=int, default=4, help="learned: held-out cases")
+    ap.add_argument("--snapshots", type=int, default=80, help="learned: snapshots per fine training run")
+    ap.add_argument("--theta", type=float, default=None, help="learned: shock-sensor threshold")
+    ap.add_argument("--train-theta", type=float, default=None,
+                    help="learned: sensor threshold that zeroes training rows (default: none)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="radial Sod: oracle steps of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

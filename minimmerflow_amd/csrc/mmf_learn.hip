@@ -529,6 +529,8 @@ __global__ __launch_bounds__(256) void k_learn_collect(GridDev fg, const double 
   if (tid == 4) {
     bool all = true;
     for (int e = 0; e < 25; ++e) all = all && sex[e / 5][e % 5];
+    // the shock sensor: largest relative jump |q_n - q0| / (q_n + q0) of rho
+    // and p to the four face neighbours
     double phi = 0.0;
     for (int a = 0; a < 2; ++a) {
       const int mj = a ? 1 : 2, mi = a ? 2 : 1, pj = a ? 3 : 2, pi = a ? 2 : 3;
@@ -545,8 +547,10 @@ __global__ __launch_bounds__(256) void k_learn_collect(GridDev fg, const double 
             q[t] = p5[FID_P];
           }
         }
-        const double val = fabs((q[2] - 2.0 * q[1]) + q[0]) / ((q[2] + 2.0 * q[1]) + q[0]);
-        phi = (val > phi) ? val : phi;
+        for (int t = 0; t < 3; t += 2) {
+          const double val = fabs(q[t] - q[1]) / (q[t] + q[1]);
+          phi = (val > phi) ? val : phi;
+        }
       }
     }
     wsh = (all && phi <= theta) ? 1.0f : 0.0f;
@@ -692,18 +696,20 @@ __device__ __forceinline__ bool cell_face(const GridDev &g, const LearnRun &run,
       }
     const double tau = TAU_REL * (fabs(reg[FID_RHO][ry][rx]) + fabs(reg[FID_RHO_E][ry][rx]));
     inv_s = (float)(1.0 / ((dev >= tau) ? dev : tau));
-    // shock sensor: lane group 0 on rho, group 1 on p; fires where
-    // |q+ - 2 q0 + q-| > theta (q+ + 2 q0 + q-) along either axis
+    // shock sensor: lane group 0 on rho, group 1 on p; fires where the jump
+    // to a face neighbour |q_n - q0| exceeds theta (q_n + q0) along either axis
     if (grp < 2) {
       const double *q = (grp == 0) ? &reg[FID_RHO][0][0] : pre;
+      const double q0 = q[ry * RX + rx];
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const double qm = a ? q[(ry - 1) * RX + rx] : q[ry * RX + rx - 1];
-        const double q0 = q[ry * RX + rx];
-        const double qp = a ? q[(ry + 1) * RX + rx] : q[ry * RX + rx + 1];
-        const double num = fabs((qp - 2.0 * q0) + qm);
-        const double den = (qp + 2.0 * q0) + qm;
-        fires = fires || !(den > 0.0) || !(num <= run.theta * den);
+#pragma unroll
+        for (int d = -1; d <= 1; d += 2) {
+          const double qn = a ? q[(ry + d) * RX + rx] : q[ry * RX + rx + d];
+          const double num = fabs(qn - q0);
+          const double den = qn + q0;
+          fires = fires || !(den > 0.0) || !(num <= run.theta * den);
+        }
       }
     }
   }

@@ -27,10 +27,11 @@ Definitions shared with the HIP kernels (every choice is this package's own, wit
   * loss sum_s w_s sum_o (yhat - t)^2 / (16 B) over a batch of B rows, with
     yhat = P + sigmoid(z) (Q - P) in normalised units;
   * Adam (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected);
-  * shock sensor: max over axes of |q+ - 2 q0 + q-| / (q+ + 2 q0 + q-) for
-    q = rho and q = p; the cell falls back to the plain face values when it
-    exceeds the threshold, when its stencil crosses a non-periodic border,
-    or when a learned face state has non-positive density or pressure;
+  * shock sensor: the largest relative jump |q_n - q0| / (q_n + q0) of
+    q = rho and q = p to the four face neighbours; the cell falls back to the
+    plain face values when it exceeds the threshold, when its stencil crosses
+    a non-periodic border, or when a learned face state has non-positive
+    density or pressure;
   * safeguards on learned values: clipped between the means of the two cells
     sharing the face, optionally blended towards the plain value;
   * targets: per coarse face, the mean of the fine grid's plain face values
@@ -219,19 +220,17 @@ def restrict(U, r):
 
 
 def sensor(U, periodic):
-    """Largest normalised second difference of rho and p over the two axes;
-    +inf where a neighbour is missing."""
+    """Largest relative jump |q_n - q0| / (q_n + q0) of rho and p to the four
+    face neighbours; +inf where a neighbour is missing."""
     q = (U[..., 0], pressure(U))
     phi = np.zeros(U.shape[:2])
     for a, axis in ((0, 1), (1, 0)):
         for Q in q:
-            Qm, em = neighbour(Q[..., None], -1, axis, periodic[a])
-            Qp, ep = neighbour(Q[..., None], +1, axis, periodic[a])
-            Qm, Qp = Qm[..., 0], Qp[..., 0]
-            num = np.abs(Qp - 2.0 * Q + Qm)
-            den = Qp + 2.0 * Q + Qm
-            val = np.where(em & ep, num / den, np.inf)
-            phi = np.maximum(phi, val)
+            for d in (-1, 1):
+                Qn, en = neighbour(Q[..., None], d, axis, periodic[a])
+                Qn = Qn[..., 0]
+                val = np.where(en, np.abs(Qn - Q) / (Qn + Q), np.inf)
+                phi = np.maximum(phi, val)
     return phi
 
 

@@ -283,3 +283,41 @@ def test_trained_network_gains_over_plain(mmf):
         print("%s: plain %.5f learned %.5f (network on %.1f%% of cell-stages)"
               % (c["kind"], ep, el, 100.0 * lc / max(1, lc + fc)), flush=True)
         assert el < (0.5 if c["kind"] == "vortex" else 0.9) * ep
+
+
+def test_learned_128_at_the_configured_size(mmf):
+    """The configured comparison at its size: train on 512^2 plain runs
+    restricted 4x (seeded vortices and radial-Sod mixes), run held-out cases
+    at 128^2 learned and plain and at 256^2 plain.  The learned 128^2 run cuts
+    the plain 128^2 error, beats plain 256^2 on the vortex, and on radial Sod
+    the shock sensor hands the shock and contact cells to the plain limiter
+    (more fallback than the two-cell border ring), while on the vortex it
+    stays quiet."""
+    from minimmerflow_amd import learned as LR
+    n = 128
+    cases = LR.case_params(16, seed=11)
+    net = mmf.Learner(seed=5, capacity=16 * 40 * 64 * 8)
+    LR.build_training_set(net, cases, 4 * n, 4, 40, 64, seed=1)
+    loss = net.train(3000, batch=8192, lr0=2e-3, lr1=2e-5, seed=1)
+    print("training rows %d, final loss %.3e" % (net.num_samples(), loss), flush=True)
+    ring = 1.0 - ((n - 4) / n) ** 2  # cells whose 5x5 stencil crosses a free-flow border
+    for c in LR.case_params(2, seed=99):
+        fine, _, _, _ = LR.run_case(c, 4 * n)
+        p256, pb2, _, _ = LR.run_case(c, 2 * n)
+        plain, pb, _, _ = LR.run_case(c, n)
+        learned, pbl, _, _ = LR.run_case(c, n, net=net)
+        e_plain = LR.l1_error(c, plain, pb, fine, 4)
+        e_256 = LR.l1_error(c, p256, pb2, fine, 2)
+        e_learned = LR.l1_error(c, learned, pbl, fine, 4)
+        lc, fc = learned.learned_counts()
+        frac = lc / max(1, lc + fc)
+        print("%s: L1 plain 128^2 %.5f, plain 256^2 %.5f, learned 128^2 %.5f; network on %.2f%% of cell-stages "
+              "(border ring %.2f%%)" % (c["kind"], e_plain, e_256, e_learned, 100 * frac, 100 * ring), flush=True)
+        assert e_learned < (0.5 if c["kind"] == "vortex" else 0.8) * e_plain
+        if c["kind"] == "vortex":
+            assert frac == 1.0  # periodic, smooth: the sensor never fires
+            assert e_learned < e_256
+        else:
+            assert frac < 1.0 - ring - 0.005  # the sensor fires beyond the border ring
+        for sv in (fine, p256, plain, learned):
+            sv.close()

@@ -221,7 +221,9 @@ def test_sensor_closed_forms():
                                  np.zeros((ny, nx)), np.zeros((ny, nx)), np.zeros((ny, nx)),
                                  np.ones((ny, nx))], axis=-1))
     phi = LO.sensor(lin, (False, False))
-    assert np.all(phi[1:-1, 1:-1] < 1e-15)
+    # rho = 1 + 0.1 i, p = 1: the largest jump is to the lower neighbour, 0.1 / (rho_{i-1} + rho_i)
+    i = np.arange(1, nx - 1)
+    assert np.allclose(phi[1:-1, 1:-1], (0.1 / (2.0 + 0.1 * (2 * i - 1)))[None, :], rtol=1e-12)
     assert np.all(np.isinf(phi[0])) and np.all(np.isinf(phi[:, 0]))
     step = np.ones((ny, nx, 5))
     step[..., 1:4] = 0.0
@@ -229,8 +231,17 @@ def test_sensor_closed_forms():
     step[:, 3:, 0] = 4.0
     step[:, 3:, 4] = 10.0
     phi = LO.sensor(step, (True, True))
-    # rho: |4 - 2 + 1| / (4 + 2 + 1) = 3/7 at the last low cell
-    assert np.isclose(phi[2, 2], 3.0 / 7.0)
+    # the largest jump at the last low cell: rho |4 - 1| / (4 + 1) = 3/5 (p jumps 1 -> 2.4, less)
+    assert phi[2, 2] == 3.0 / 5.0
+    # two cells away from the step the field is flat: nothing fires
+    assert phi[2, 1] == 0.0
+    # p = rho T with T = 1 (rho E = 2.5 rho at rest): the pressure jump equals the density jump
+    wave = np.ones((ny, nx, 5))
+    wave[..., 1:4] = 0.0
+    wave[..., 0] = 1.0 + 0.01 * np.arange(nx)[None, :]
+    wave[..., 4] = 2.5 * wave[..., 0]
+    phi = LO.sensor(wave, (False, False))
+    assert np.allclose(phi[1:-1, 1:-1], (0.01 / (2.0 + 0.01 * (2 * i - 1)))[None, :], rtol=1e-12)
 
 
 def test_collect_weights_and_zero_network():
