@@ -113,6 +113,22 @@ int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nboxes,
 /* flags: [ncells] MMF_CELL_* bits */
 int mmf_get_cell_flags(mmf_ctx *ctx, uint8_t *flags);
 
+/* ---- border conditions per border face (main.cpp:251-260) ---------------- */
+/* The reference keeps a BC code per border interface (interfaceBCs,
+ * main.cpp:251-260, from problem::getBorderBCType at the face centroid) and
+ * Dirichlet data per face (euler.cpp:261-268).  Entries side by side: x-, x+,
+ * y-, y+ (, z-, z+); a side's faces ordered by its two other axes, the lower
+ * one fastest (side y- of a 3D grid: i + n_x k).  mmf_border_face_count gives
+ * the entry count of a descriptor's grid.  bc: MMF_BC_FREE_FLOW ..
+ * MMF_BC_DIRICHLET; info: [count][5] Dirichlet primitives (p,u,v,w,T), read
+ * for Dirichlet entries only, may be NULL when there is none.  Entries of
+ * sides on periodic axes are ignored.  A side whose entries agree becomes a
+ * per-side condition; with a side that does not (at most 256 distinct
+ * conditions), the stage kernel is the per-cell gather kernel, the tile,
+ * marching and learned kernels do not apply. */
+int mmf_border_face_count(const mmf_grid_desc *desc, int64_t *count);
+int mmf_set_border_bcs(mmf_ctx *ctx, const int32_t *bc, const double *info);
+
 /* ---- state transfer (AoS host <-> SoA device) --------------------------- */
 int mmf_set_state(mmf_ctx *ctx, const double *U_aos);
 int mmf_get_state(mmf_ctx *ctx, double *U_aos);
@@ -266,8 +282,16 @@ int mmf_problem_exact(const mmf_problem *pb, const double *points, int64_t count
 /* problem::getBorderBCType / getBorderBCInfo (problem.cpp:404-477) */
 int mmf_problem_border_bc(const mmf_problem *pb, const double *face_centroid, int32_t *bc,
                           double *info);
-/* side_bc / side_info of desc from the problem's border rule */
+/* side_bc / side_info of desc from the problem's border rule, evaluated at
+ * one face per side (the side's centre; FFSTEP, whose rule depends on x only:
+ * a face of the first column).  Exact for every problem whose rule is uniform
+ * per side; FFSTEP's y and z sides change condition at x = 3.2 (problem.cpp:
+ * 422-432), which mmf_problem_border_bcs + mmf_set_border_bcs represent. */
 int mmf_problem_side_bcs(const mmf_problem *pb, mmf_grid_desc *desc);
+/* problem.cpp:404-477 at the centroid of every border face of desc's grid
+ * (main.cpp:251-260), in mmf_set_border_bcs's layout; info may be NULL */
+int mmf_problem_border_bcs(const mmf_problem *pb, const mmf_grid_desc *desc, int32_t *bc,
+                           double *info);
 /* main.cpp:335-344: upload the exact solution at t=0 sampled at centroids */
 int mmf_set_initial_state(mmf_ctx *ctx, const mmf_problem *pb);
 /* main.cpp:550-561 against the problem's exact density at time t */

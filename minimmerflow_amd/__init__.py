@@ -131,6 +131,9 @@ def lib():
         "mmf_problem_exact": ([C.POINTER(Problem), dp, C.c_int64, C.c_double, dp], C.c_int),
         "mmf_problem_border_bc": ([C.POINTER(Problem), dp, i32p, dp], C.c_int),
         "mmf_problem_side_bcs": ([C.POINTER(Problem), C.POINTER(GridDesc)], C.c_int),
+        "mmf_problem_border_bcs": ([C.POINTER(Problem), C.POINTER(GridDesc), i32p, dp], C.c_int),
+        "mmf_border_face_count": ([C.POINTER(GridDesc), i64p], C.c_int),
+        "mmf_set_border_bcs": ([vp, i32p, dp], C.c_int),
         "mmf_set_initial_state": ([vp, C.POINTER(Problem)], C.c_int),
         "mmf_error_vs_exact": ([vp, C.POINTER(Problem), C.c_double, dp, dp], C.c_int),
         "mmf_get_centroids": ([vp, dp], C.c_int),
@@ -355,7 +358,13 @@ class Solver:
         _check(lib().mmf_problem_side_bcs(C.byref(pb), C.byref(d)))
         bcs = [d.side_bc[s] for s in range(6)] if side_bc is None else list(side_bc)
         info = [[d.side_info[s][k] for k in range(NF)] for s in range(6)]
-        return cls(dim, n, origin, h, bcs, info, order)
+        s = cls(dim, n, origin, h, bcs, info, order)
+        if side_bc is None:
+            # a side whose faces differ (FFSTEP past x = 3.2) gets them face by face
+            fb, fi = problem_border_bcs(pb, d)
+            if not _sides_uniform(d, fb, fi):
+                s.set_border_bcs(fb, fi)
+        return s
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -422,6 +431,17 @@ class Solver:
         s = _f64(np.asarray(spheres, dtype=np.float64).reshape(-1, 4))
         self._c(lib().mmf_set_bodies(self.ctx, _dp(b) if len(b) else None, len(b),
                                      _dp(s) if len(s) else None, len(s)))
+
+    def set_border_bcs(self, bc, info=None):
+        """Border condition of every border face (main.cpp:251-260), in the
+        layout of border_face_count: sides x-, x+, y-, y+ (, z-, z+), a side's
+        faces ordered by its two other axes, the lower one fastest."""
+        b = np.ascontiguousarray(bc, dtype=np.int32).ravel()
+        if len(b) != border_face_count(self.desc):
+            raise ValueError("one border condition per border face: %d" % border_face_count(self.desc))
+        inf = None if info is None else _f64(np.asarray(info, dtype=np.float64).reshape(len(b), NF))
+        self._c(lib().mmf_set_border_bcs(self.ctx, b.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         _dp(inf) if inf is not None else None))
 
     def cell_flags(self):
         out = np.empty(self.ncells, dtype=np.uint8)
@@ -551,6 +571,37 @@ def grid_desc(dim, n, origin, h, side_bc, side_info=None, order=1):
     return d
 
 
+def border_face_count(desc):
+    """Border faces of a descriptor's grid (the entries of set_border_bcs)."""
+    c = C.c_int64()
+    _check(lib().mmf_border_face_count(C.byref(desc), C.byref(c)))
+    return c.value
+
+
+def problem_border_bcs(pb, desc):
+    """problem.cpp:404-477 at every border face centroid of desc's grid:
+    (codes [count], Dirichlet primitives [count][5])."""
+    m = border_face_count(desc)
+    bc = np.empty(m, dtype=np.int32)
+    info = np.empty((m, NF))
+    _check(lib().mmf_problem_border_bcs(C.byref(pb), C.byref(desc), bc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        _dp(info)))
+    return bc, info
+
+
+def _sides_uniform(desc, bc, info):
+    n = [desc.n[a] for a in range(3)]
+    off = 0
+    for s in range(2 * desc.dim):
+        a = s // 2
+        cnt = n[1] * n[2] if a == 0 else (n[0] * n[2] if a == 1 else n[0] * n[1])
+        b, i = bc[off:off + cnt], info[off:off + cnt]
+        if desc.side_bc[2 * a] != BC_PERIODIC and (np.any(b != b[0]) or np.any(i != i[0])):
+            return False
+        off += cnt
+    return True
+
+
 def problem_grid_desc(pb, ncells, order=1):
     """Grid of computation() for a problem (main.cpp:147-149, 251-260)."""
     d = GridDesc()
@@ -559,6 +610,9 @@ def problem_grid_desc(pb, ncells, order=1):
         d.n[a] = int(ncells) if a < pb.dim else 1
         d.origin[a] = pb.domain_origin[a]
     _check(lib().mmf_problem_side_bcs(C.byref(pb), C.byref(d)))
+    if not _sides_uniform(d, *problem_border_bcs(pb, d)):
+        raise MMFError(ERR_INVALID, "the problem's border conditions change along a side; "
+                                    "a descriptor holds one per side (Solver.for_problem applies them per face)")
     return d
 
 

@@ -105,6 +105,15 @@ int run(int argc, char **argv) {
   int rc = mmf_create(&d, &ctx);
   if (rc == MMF_ERR_UNSUPPORTED_ORDER) return 2;  // reconstruction.cpp:75-76: exit(2)
   check(rc, ctx, "create");
+  {
+    // main.cpp:251-260: every border interface's condition at its centroid
+    int64_t nb = 0;
+    check(mmf_border_face_count(&d, &nb), ctx, "border conditions");
+    std::vector<int32_t> fbc((size_t)nb);
+    std::vector<double> finfo((size_t)(5 * nb));
+    check(mmf_problem_border_bcs(&pb, &d, fbc.data(), finfo.data()), ctx, "border conditions");
+    check(mmf_set_border_bcs(ctx, fbc.data(), finfo.data()), ctx, "border conditions");
+  }
   if (!boxes.empty()) check(mmf_set_bodies(ctx, boxes.data(), (int32_t)(boxes.size() / 6), nullptr, 0), ctx, "bodies");
   check(mmf_set_initial_state(ctx, &pb), ctx, "initial conditions");
   check(mmf_set_time(ctx, tMin, tMax, cfl), ctx, "time");

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <new>
 #include <string>
 #include <utility>
@@ -57,7 +58,9 @@ bool use_march(const mmf_ctx *ctx) {
   return true;
 }
 
-bool tile_applies(const mmf_ctx *ctx) { return ctx->g.dim == 2 && ctx->g.order == 2 && !ctx->g.flags; }
+bool tile_applies(const mmf_ctx *ctx) {
+  return ctx->g.dim == 2 && ctx->g.order == 2 && !ctx->g.flags && !ctx->g.bslot;
+}
 
 // AUTO takes the tile kernel for 2D order-2 grids below this size (measured on
 // MI355X, vortex: 128^2 25.7 vs 50.0 us per step for the gather kernel, 256^2
@@ -285,6 +288,9 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->dlcount) (void)hipFree(ctx->dlcount);
   if (ctx->draw) (void)hipFree(ctx->draw);
   if (ctx->dtiles) (void)hipFree(ctx->dtiles);
+  if (ctx->dbslot) (void)hipFree(ctx->dbslot);
+  if (ctx->dbslot_bc) (void)hipFree(ctx->dbslot_bc);
+  if (ctx->dbslot_info) (void)hipFree(ctx->dbslot_info);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMF_OK;
@@ -372,6 +378,108 @@ int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved
   HIPCHK(ctx, hipMemcpy(ctx->dflags, f.data(), (size_t)N, hipMemcpyHostToDevice));
   ctx->g.flags = ctx->dflags;
   ctx->hflags.swap(f);
+  return MMF_OK;
+}
+
+int mmf_border_face_count(const mmf_grid_desc *desc, int64_t *count) {
+  GridDev g;
+  const int rc = mmf::grid_from_desc(desc, g);
+  if (rc != MMF_OK) return rc;
+  if (!count) return MMF_ERR_INVALID;
+  int64_t c = 0;
+  for (int s = 0; s < 2 * g.dim; ++s) c += mmf::border_faces_of_side(g.n, s);
+  *count = c;
+  return MMF_OK;
+}
+
+// main.cpp:251-260 (interfaceBCs per border interface).  Each distinct
+// (code, Dirichlet data) becomes a slot; a side whose faces share one slot
+// keeps the per-side form, which every stage kernel reads.
+int mmf_set_border_bcs(mmf_ctx *ctx, const int32_t *bc, const double *info) {
+  if (!ctx || !bc) return MMF_ERR_INVALID;
+  if (ctx->learn_on) return fail(ctx, MMF_ERR_STATE, "border conditions: the learned reconstruction is on");
+  GridDev &g = ctx->g;
+  int64_t off[6] = {0, 0, 0, 0, 0, 0}, total = 0;
+  for (int s = 0; s < 2 * g.dim; ++s) {
+    off[s] = total;
+    total += mmf::border_faces_of_side(g.n, s);
+  }
+  std::map<std::vector<uint64_t>, int> slot_of;
+  std::vector<int32_t> sbc;
+  std::vector<double> sinfo;
+  std::vector<uint8_t> slot((size_t)total, 0);
+  bool uniform[6] = {true, true, true, true, true, true};
+  for (int s = 0; s < 2 * g.dim; ++s) {
+    if (g.periodic[s / 2]) continue;  // interior faces: entries ignored
+    const int64_t nfs = mmf::border_faces_of_side(g.n, s);
+    for (int64_t e = off[s]; e < off[s] + nfs; ++e) {
+      const int32_t b = bc[e];
+      if (b < MMF_BC_FREE_FLOW || b > MMF_BC_DIRICHLET) return fail(ctx, MMF_ERR_INVALID, "border conditions: code");
+      if (b == MMF_BC_DIRICHLET && !info) return fail(ctx, MMF_ERR_INVALID, "border conditions: Dirichlet data");
+      std::vector<uint64_t> key(1 + 5, 0);
+      key[0] = (uint64_t)b;
+      if (b == MMF_BC_DIRICHLET) std::memcpy(&key[1], info + 5 * e, 5 * sizeof(double));
+      auto it = slot_of.find(key);
+      int k;
+      if (it == slot_of.end()) {
+        k = (int)sbc.size();
+        if (k > 255) return fail(ctx, MMF_ERR_INVALID, "border conditions: more than 256 distinct conditions");
+        slot_of.emplace(key, k);
+        sbc.push_back(b);
+        for (int q = 0; q < 5; ++q) sinfo.push_back(b == MMF_BC_DIRICHLET ? info[5 * e + q] : 0.0);
+      } else {
+        k = it->second;
+      }
+      slot[(size_t)e] = (uint8_t)k;
+      if (slot[(size_t)e] != slot[(size_t)off[s]]) uniform[s] = false;
+    }
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // no launch in flight reads the old table
+  ctx->lam_valid = false;
+  ctx->tiles_valid = false;
+  bool all_uniform = true;
+  for (int s = 0; s < 2 * g.dim; ++s) {
+    if (g.periodic[s / 2]) continue;
+    const int k = slot[(size_t)off[s]];
+    g.bc[s] = sbc[(size_t)k];  // a non-uniform side: its first face's (not read)
+    for (int q = 0; q < 5; ++q) g.info[s][q] = sinfo[(size_t)(5 * k + q)];
+    if (uniform[s]) {
+      ctx->desc.side_bc[s] = g.bc[s];
+      for (int q = 0; q < 5; ++q) ctx->desc.side_info[s][q] = g.info[s][q];
+    }
+    all_uniform = all_uniform && uniform[s];
+  }
+  if (ctx->dbslot) (void)hipFree(ctx->dbslot);
+  if (ctx->dbslot_bc) (void)hipFree(ctx->dbslot_bc);
+  if (ctx->dbslot_info) (void)hipFree(ctx->dbslot_info);
+  ctx->dbslot = nullptr;
+  ctx->dbslot_bc = nullptr;
+  ctx->dbslot_info = nullptr;
+  g.bslot = nullptr;
+  g.bslot_bc = nullptr;
+  g.bslot_info = nullptr;
+  for (int s = 0; s < 6; ++s) g.border_off[s] = 0;
+  if (all_uniform) return MMF_OK;
+  hipError_t e = hipMalloc(&ctx->dbslot, (size_t)total);
+  if (e == hipSuccess) e = hipMalloc(&ctx->dbslot_bc, sizeof(int32_t) * sbc.size());
+  if (e == hipSuccess) e = hipMalloc(&ctx->dbslot_info, sizeof(double) * sinfo.size());
+  if (e == hipSuccess) e = hipMemcpy(ctx->dbslot, slot.data(), (size_t)total, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ctx->dbslot_bc, sbc.data(), sizeof(int32_t) * sbc.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(ctx->dbslot_info, sinfo.data(), sizeof(double) * sinfo.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (ctx->dbslot) (void)hipFree(ctx->dbslot);
+    if (ctx->dbslot_bc) (void)hipFree(ctx->dbslot_bc);
+    if (ctx->dbslot_info) (void)hipFree(ctx->dbslot_info);
+    ctx->dbslot = nullptr;
+    ctx->dbslot_bc = nullptr;
+    ctx->dbslot_info = nullptr;
+    return hip_fail(ctx, e, "border conditions");
+  }
+  g.bslot = ctx->dbslot;
+  g.bslot_bc = ctx->dbslot_bc;
+  g.bslot_info = ctx->dbslot_info;
+  for (int s = 0; s < 6; ++s) g.border_off[s] = off[s];
   return MMF_OK;
 }
 

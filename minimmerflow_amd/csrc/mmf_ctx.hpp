@@ -58,6 +58,10 @@ struct mmf_ctx {
   int32_t *dtiles = nullptr;
   int64_t dtiles_n = 0;
   bool tiles_valid = false;
+  // per-face border conditions (GridDev::bslot and its slot tables)
+  uint8_t *dbslot = nullptr;
+  int32_t *dbslot_bc = nullptr;
+  double *dbslot_info = nullptr;
   double *dFS = nullptr;
   unsigned long long *dlcount = nullptr;
   mmf_output *out = nullptr;

@@ -30,7 +30,23 @@ struct GridDev {
   // 3D marching kernel on a grid with flags: per block, its tile, with
   // MARCH_TILE_MASKED set where the block must read the flags; or nullptr
   const int32_t *march_tiles;
+  // Border conditions per border face (main.cpp:251-260 keeps one per
+  // interface), or nullptr when bc / info above hold per side: the slot of
+  // each border face, sides one after another from border_off[side], a
+  // side's faces ordered by its two other axes, the lower one fastest; each
+  // slot's BC code and Dirichlet primitives.  Only the per-cell gather kernel
+  // reads it.
+  const uint8_t *bslot;
+  const int32_t *bslot_bc;
+  const double *bslot_info;
+  int64_t border_off[6];
 };
+
+// Border faces of side 'side' (product of the two other axes' counts).
+inline int64_t border_faces_of_side(const int64_t *n, int side) {
+  const int a = side / 2;
+  return (a == 0 ? n[1] * n[2] : a == 1 ? n[0] * n[2] : n[0] * n[1]);
+}
 
 constexpr int32_t MARCH_TILE_MASKED = 1 << 30;
 

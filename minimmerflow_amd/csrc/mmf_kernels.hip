@@ -65,10 +65,10 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
           // lower face: R = this cell
 #pragma unroll
           for (int k = 0; k < NF; ++k) Rm[k] = Xc[k], Lp[k] = Xc[k];
-          if (!em1) border_state(g, 2 * a, a, Xc, Lm);
+          if (!em1) border_state(g, 2 * a, a, q, Xc, Lm);
           else if (wall_m) reflect_axis(Xc, a, Lm);
           else in.load(g, nm1, Lm);
-          if (!ep1) border_state(g, 2 * a + 1, a, Xc, Rp);
+          if (!ep1) border_state(g, 2 * a + 1, a, q, Xc, Rp);
           else if (wall_p) reflect_axis(Xc, a, Rp);
           else in.load(g, np1, Rp);
         } else {
@@ -125,10 +125,10 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
           };
           recon(Xc, sc, -1.0, Rm);
           recon(Xc, sc, +1.0, Lp);
-          if (!em1) border_state(g, 2 * a, a, Rm, Lm);
+          if (!em1) border_state(g, 2 * a, a, q, Rm, Lm);
           else if (wall_m) reflect_axis(Rm, a, Lm);
           else recon(Xm1, sm, +1.0, Lm);
-          if (!ep1) border_state(g, 2 * a + 1, a, Lp, Rp);
+          if (!ep1) border_state(g, 2 * a + 1, a, q, Lp, Rp);
           else if (wall_p) reflect_axis(Lp, a, Rp);
           else recon(Xp1, sp, -1.0, Rp);
         }

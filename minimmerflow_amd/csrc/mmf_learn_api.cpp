@@ -316,8 +316,8 @@ int mmf_set_learned_reconstruction(mmf_ctx *ctx, const mmf_learn *n, double sens
     ctx->learn_on = false;
     return MMF_OK;
   }
-  if (ctx->g.dim != 2 || ctx->g.order != 2 || ctx->g.flags) {
-    ctx->err = "learned reconstruction: 2D order-2 grids without bodies";
+  if (ctx->g.dim != 2 || ctx->g.order != 2 || ctx->g.flags || ctx->g.bslot) {
+    ctx->err = "learned reconstruction: 2D order-2 grids without bodies or per-face border conditions";
     return MMF_ERR_INVALID;
   }
   CCHK(ctx, hipStreamSynchronize(ctx->stream));

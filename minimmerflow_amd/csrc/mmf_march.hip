@@ -726,7 +726,7 @@ void build_march_tiles(const GridDev &g, const MarchShape &sh, const uint8_t *fl
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
-  if (sh.chunk < 1 || g.N >= MARCH_MAX_CELLS) return false;
+  if (sh.chunk < 1 || g.N >= MARCH_MAX_CELLS || g.bslot) return false;  // per-face borders: the gather kernel
   // order 2: the 2D strip kernel, grids without bodies
   if (g.order != 1 && !(g.order == 2 && g.dim == 2 && g.flags == nullptr)) return false;
   if (g.dim == 2) return sh.ty == 1;

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mmf.h"
+#include "mmf_internal.hpp"
 
 namespace {
 
@@ -163,8 +164,11 @@ int mmf_problem_border_bc(const mmf_problem *pb, const double *face_centroid, in
 }
 
 // Fills desc->side_bc / side_info from the problem: the per-face rule of
-// problem.cpp:404-477 evaluated at the centre face of each side.  FFSTEP's rule
-// depends on x only, so it is uniform per side when the domain ends at x<=3.2.
+// problem.cpp:404-477 evaluated at one face of each side (the centre of the
+// side; for FFSTEP, whose rule depends on x only, a face of the first
+// column).  A side whose faces do not share one condition (FFSTEP's y and z
+// sides on a domain reaching past x = 3.2) takes that face's here;
+// mmf_problem_border_bcs gives every face its own.
 int mmf_problem_side_bcs(const mmf_problem *pb, mmf_grid_desc *desc) {
   if (!pb || !desc) return MMF_ERR_INVALID;
   for (int s = 0; s < 6; ++s) {
@@ -177,6 +181,34 @@ int mmf_problem_side_bcs(const mmf_problem *pb, mmf_grid_desc *desc) {
     int rc = mmf_problem_border_bc(pb, fc, &bc, desc->side_info[s]);
     if (rc != MMF_OK) return rc;
     desc->side_bc[s] = bc;
+  }
+  return MMF_OK;
+}
+
+// problem.cpp:404-477 at the centroid of every border face of desc's grid
+// (main.cpp:251-260), in mmf_set_border_bcs's layout: sides x-, x+, y-, y+
+// (, z-, z+), a side's faces ordered by its two other axes, the lower one
+// fastest; face centroid = origin + (i + 1/2) h along those axes, the origin
+// or origin + n h across (z = origin z in 2D).
+int mmf_problem_border_bcs(const mmf_problem *pb, const mmf_grid_desc *desc, int32_t *bc, double *info) {
+  if (!pb || !bc) return MMF_ERR_INVALID;
+  mmf::GridDev g;
+  const int rcg = mmf::grid_from_desc(desc, g);
+  if (rcg != MMF_OK) return rcg;
+  int64_t e = 0;
+  for (int s = 0; s < 2 * g.dim; ++s) {
+    const int a = s / 2;
+    const int b0 = (a == 0) ? 1 : 0, b1 = (a == 2) ? 1 : 2;
+    for (int64_t j = 0; j < g.n[b1]; ++j)
+      for (int64_t i = 0; i < g.n[b0]; ++i, ++e) {
+        double fc[3];
+        for (int d = 0; d < 3; ++d) fc[d] = desc->origin[d];
+        if (b0 < g.dim) fc[b0] = desc->origin[b0] + ((double)i + 0.5) * desc->h;
+        if (b1 < g.dim) fc[b1] = desc->origin[b1] + ((double)j + 0.5) * desc->h;
+        fc[a] = desc->origin[a] + ((s & 1) ? (double)g.n[a] : 0.0) * desc->h;
+        const int rc = mmf_problem_border_bc(pb, fc, &bc[e], info ? info + 5 * e : nullptr);
+        if (rc != MMF_OK) return rc;
+      }
   }
   return MMF_OK;
 }

@@ -217,16 +217,36 @@ __device__ __forceinline__ double minmod(double a, double b) {
   return 0.0;
 }
 
-// Border virtual state (euler.cpp:261-376) from the fluid-side face value.
-__device__ __forceinline__ void border_state(const GridDev &g, int side, int a, const double *fluid, double *out) {
-  int bc = g.bc[side];
+// Border virtual state (euler.cpp:261-376) of condition bc (Dirichlet data
+// info) from the fluid-side face value.
+__device__ __forceinline__ void border_state_bc(int bc, const double *info, int a, const double *fluid, double *out) {
   if (bc == BC_DIRICHLET) {
-    prim2cons(g.info[side], out);
+    prim2cons(info, out);
   } else if (bc == BC_REFLECTING || bc == BC_WALL) {
     reflect_axis(fluid, a, out);
   } else {
 #pragma unroll
     for (int k = 0; k < NF; ++k) out[k] = fluid[k];
+  }
+}
+
+// The same with the side's condition (kernels that run on per-side
+// conditions only).
+__device__ __forceinline__ void border_state(const GridDev &g, int side, int a, const double *fluid, double *out) {
+  border_state_bc(g.bc[side], g.info[side], a, fluid, out);
+}
+
+// The same for the border face of cell q on side 'side' (axis a): its own
+// condition when the grid has per-face conditions.
+__device__ __forceinline__ void border_state(const GridDev &g, int side, int a, const int64_t *q,
+                                             const double *fluid, double *out) {
+  if (g.bslot) {
+    // the face's index on the side from the two other axes, the lower fastest
+    const int b0 = (a == 0) ? 1 : 0, b1 = (a == 2) ? 1 : 2;
+    const int s = g.bslot[g.border_off[side] + q[b0] + g.n[b0] * q[b1]];
+    border_state_bc(g.bslot_bc[s], g.bslot_info + NF * s, a, fluid, out);
+  } else {
+    border_state_bc(g.bc[side], g.info[side], a, fluid, out);
   }
 }
 
