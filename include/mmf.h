@@ -142,6 +142,45 @@ int mmf_set_raw_order(mmf_ctx *ctx, const int64_t *cell_of_raw, int64_t nraw);
  * reconstruction::computePolynomials hook, reconstruction.cpp:47-55).
  * rhs_aos: host [ncells][5] or NULL; max_eig: host scalar or NULL. */
 int mmf_compute_rhs(mmf_ctx *ctx, double *rhs_aos, double *max_eig);
+/* euler::computeRHS with the arguments the reference passes on every call
+ * (euler.hpp:41-43): solved [ncells] 0/1 (host order; NULL keeps the
+ * context's), order 1 or 2 (else MMF_ERR_UNSUPPORTED_ORDER, the reference's
+ * exit(2), reconstruction.cpp:75-76), U_aos the state (NULL: the current
+ * one).  Flags and order are re-sent only where they differ from the
+ * context's, so a caller may change them between calls (ghost cells
+ * unsolved, main.cpp:231-235).  The border conditions (interfaceBCs) are
+ * the context's: mmf_set_border_bcs; fluid-solid faces follow the fluid
+ * flags (main.cpp:271-275). */
+int mmf_compute_rhs_ex(mmf_ctx *ctx, const uint8_t *solved, int32_t order, const double *U_aos,
+                       double *rhs_aos, double *max_eig);
+
+/* ---- one RK3 stage at a time (main.cpp:377-506) ----------------------------- */
+/* For a caller that runs something between the stages of a step: the ghost
+ * exchange of a partitioned mesh (main.cpp:425-430, 461-466, 497-502) or the
+ * reduction of the eigenvalue over ranks (main.cpp:390-395).  One step is
+ *   mmf_rk_rhs(ctx, MMF_STATE_U, &e);  [reduce e over ranks]
+ *   mmf_rk_dt(ctx, e, &dt);             main.cpp:398-402
+ *   mmf_rk_update(ctx, 1, dt);          W = U + dt R / V        [exchange W]
+ *   mmf_rk_rhs(ctx, MMF_STATE_W, &e);
+ *   mmf_rk_update(ctx, 2, dt);          W = 3/4 U + 1/4 (W + dt R / V)  [exchange W]
+ *   mmf_rk_rhs(ctx, MMF_STATE_W, &e);
+ *   mmf_rk_update(ctx, 3, dt);          U = 1/3 U + 2/3 (W + dt R / V), t += dt  [exchange U]
+ * and gives the bits of mmf_advance(ctx, 1) on the same grid (the fused
+ * kernels evaluate the same expressions).  Unsolved cells keep U; stages 1
+ * and 2 copy it into W. */
+enum { MMF_STATE_U = 0, MMF_STATE_W = 1, MMF_STATE_RHS = 2 };
+/* R = RHS(state) (state U or W); max_eig: the stage maximum (host) */
+int mmf_rk_rhs(mmf_ctx *ctx, int32_t state, double *max_eig);
+/* dt of the step from max_eig (MMF_ERR_STATE unless finite and positive) */
+int mmf_rk_dt(mmf_ctx *ctx, double max_eig, double *dt);
+int mmf_rk_update(mmf_ctx *ctx, int32_t stage, double dt);
+/* Ghost lists (communications.tpp:164-199): rows cells[n] (host order, as
+ * mmf_set_state) of a state to / from host AoS buf [n][5]. */
+int mmf_pack_cells(mmf_ctx *ctx, int32_t state, const int64_t *cells, int64_t n, double *buf);
+int mmf_unpack_cells(mmf_ctx *ctx, int32_t state, const int64_t *cells, int64_t n, const double *buf);
+/* Device SoA view of a state (field k at ptr + k * field_stride, cell order),
+ * for exchanges that stay on the device. */
+int mmf_get_device_state(mmf_ctx *ctx, int32_t state, double **ptr, int64_t *field_stride);
 
 /* ---- the same RHS, allocating nothing -------------------------------------- */
 /* No counterpart in the reference: a variant of mmf_compute_rhs for callers

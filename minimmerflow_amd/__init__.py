@@ -31,6 +31,9 @@ PROBLEM_KEYS = {"vortex_xy": VORTEX_XY, "vortex_zx": VORTEX_ZX, "vortex_yz": VOR
                 "ffstep": FFSTEP, "freestream": FREESTREAM}
 
 
+STATE_U, STATE_W, STATE_RHS = 0, 1, 2
+
+
 class MMFError(RuntimeError):
     def __init__(self, status, msg=""):
         self.status = status
@@ -134,6 +137,13 @@ def lib():
         "mmf_problem_border_bcs": ([C.POINTER(Problem), C.POINTER(GridDesc), i32p, dp], C.c_int),
         "mmf_border_face_count": ([C.POINTER(GridDesc), i64p], C.c_int),
         "mmf_set_border_bcs": ([vp, i32p, dp], C.c_int),
+        "mmf_compute_rhs_ex": ([vp, u8p, C.c_int32, dp, dp, dp], C.c_int),
+        "mmf_rk_rhs": ([vp, C.c_int32, dp], C.c_int),
+        "mmf_rk_dt": ([vp, C.c_double, dp], C.c_int),
+        "mmf_rk_update": ([vp, C.c_int32, C.c_double], C.c_int),
+        "mmf_pack_cells": ([vp, C.c_int32, i64p, C.c_int64, dp], C.c_int),
+        "mmf_unpack_cells": ([vp, C.c_int32, i64p, C.c_int64, dp], C.c_int),
+        "mmf_get_device_state": ([vp, C.c_int32, C.POINTER(C.c_void_p), i64p], C.c_int),
         "mmf_set_initial_state": ([vp, C.POINTER(Problem)], C.c_int),
         "mmf_error_vs_exact": ([vp, C.POINTER(Problem), C.c_double, dp, dp], C.c_int),
         "mmf_get_centroids": ([vp, dp], C.c_int),
@@ -454,6 +464,48 @@ class Solver:
         me = C.c_double()
         self._c(lib().mmf_compute_rhs(self.ctx, _dp(R) if want_rhs else None, C.byref(me)))
         return R, me.value
+
+    def compute_rhs_ex(self, solved, order, U=None):
+        """euler::computeRHS with the reference's per-call arguments: solved
+        flags (None keeps the context's) and order (euler.hpp:41-43)."""
+        sv = None if solved is None else np.ascontiguousarray(solved, dtype=np.uint8)
+        u = None if U is None else _f64(U)
+        R = np.empty((self.ncells, NF))
+        me = C.c_double()
+        self._c(lib().mmf_compute_rhs_ex(self.ctx, None if sv is None else sv.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                         int(order), None if u is None else _dp(u), _dp(R), C.byref(me)))
+        return R, me.value
+
+    # one RK3 stage at a time
+    def rk_rhs(self, state=0):
+        me = C.c_double()
+        self._c(lib().mmf_rk_rhs(self.ctx, int(state), C.byref(me)))
+        return me.value
+
+    def rk_dt(self, max_eig):
+        dt = C.c_double()
+        self._c(lib().mmf_rk_dt(self.ctx, float(max_eig), C.byref(dt)))
+        return dt.value
+
+    def rk_update(self, stage, dt):
+        self._c(lib().mmf_rk_update(self.ctx, int(stage), float(dt)))
+
+    def pack_cells(self, state, cells):
+        c = np.ascontiguousarray(cells, dtype=np.int64)
+        buf = np.empty((len(c), NF))
+        self._c(lib().mmf_pack_cells(self.ctx, int(state), c.ctypes.data_as(C.POINTER(C.c_int64)), len(c), _dp(buf)))
+        return buf
+
+    def unpack_cells(self, state, cells, buf):
+        c = np.ascontiguousarray(cells, dtype=np.int64)
+        b = _f64(np.asarray(buf, dtype=np.float64).reshape(len(c), NF))
+        self._c(lib().mmf_unpack_cells(self.ctx, int(state), c.ctypes.data_as(C.POINTER(C.c_int64)), len(c), _dp(b)))
+
+    def device_state(self, state):
+        """(device address, field stride) of a state's SoA planes."""
+        p, fs = C.c_void_p(), C.c_int64()
+        self._c(lib().mmf_get_device_state(self.ctx, int(state), C.byref(p), C.byref(fs)))
+        return p.value, fs.value
 
     # time loop
     def set_time(self, t, tmax, cfl):

@@ -77,7 +77,9 @@ int kernel_used(const mmf_ctx *ctx) {
   return use_march(ctx) ? mmf::KERNEL_MARCH : mmf::KERNEL_GATHER;
 }
 
-int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
+// One stage launch on states U (stage input of stage 1 and the RK base), R and
+// W: stage 1 writes R, stage 2 W, stage 3 U.
+int launch_stage_on(mmf_ctx *ctx, int stage, int mode, double *U, double *R, double *W) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->timing) {
     HIPCHK(ctx, hipEventCreate(&e0));
@@ -88,22 +90,19 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
   const int kind = kernel_used(ctx);
   if (ctx->learn_on && kind == mmf::KERNEL_GATHER) {
     // face states (learned or plain) through HBM, then flux + RK stage update
-    e = mmf::launch_recon_learned(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dctrl, ctx->stream);
+    e = mmf::launch_recon_learned(ctx->g, ctx->learn, stage, mode, U, R, W, ctx->dctrl, ctx->stream);
     if (e == hipSuccess)
-      e = mmf::launch_flux_fs(ctx->g, ctx->learn, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
-                              ctx->dctrl, ctx->stream);
+      e = mmf::launch_flux_fs(ctx->g, ctx->learn, stage, mode, U, R, W, R, W, U, ctx->dctrl, ctx->stream);
   } else if (kind == mmf::KERNEL_TILE) {
     // one launch per stage, face states in LDS; plain face values without a network
     mmf::LearnRun plain{};
     plain.blend = 1.0;
-    e = mmf::launch_stage_learned(ctx->g, ctx->learn_on ? ctx->learn : plain, stage, mode, ctx->dU, ctx->dR, ctx->dW,
-                                  ctx->dR, ctx->dW, ctx->dU, ctx->dctrl, ctx->stream);
+    e = mmf::launch_stage_learned(ctx->g, ctx->learn_on ? ctx->learn : plain, stage, mode, U, R, W, R, W, U, ctx->dctrl,
+                                  ctx->stream);
   } else if (kind == mmf::KERNEL_MARCH) {
-    e = mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU,
-                                ctx->dctrl, ctx->stream);
+    e = mmf::launch_stage_march(ctx->g, ctx->shape, stage, mode, U, R, W, R, W, U, ctx->dctrl, ctx->stream);
   } else {
-    e = mmf::launch_stage(ctx->g, stage, mode, ctx->dU, ctx->dR, ctx->dW, ctx->dR, ctx->dW, ctx->dU, ctx->dctrl,
-                          ctx->stream);
+    e = mmf::launch_stage(ctx->g, stage, mode, U, R, W, R, W, U, ctx->dctrl, ctx->stream);
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "stage kernel launch");
   if (ctx->timing) {
@@ -111,6 +110,10 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
     ctx->events.emplace_back(e0, e1);
   }
   return MMF_OK;
+}
+
+int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
+  return launch_stage_on(ctx, stage, mode, ctx->dU, ctx->dR, ctx->dW);
 }
 
 int collect_timing(mmf_ctx *ctx) {
@@ -291,6 +294,7 @@ int mmf_destroy(mmf_ctx *ctx) {
   if (ctx->dbslot) (void)hipFree(ctx->dbslot);
   if (ctx->dbslot_bc) (void)hipFree(ctx->dbslot_bc);
   if (ctx->dbslot_info) (void)hipFree(ctx->dbslot_info);
+  if (ctx->dpack) (void)hipFree(ctx->dpack);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMF_OK;
@@ -617,6 +621,147 @@ int mmf_compute_rhs(mmf_ctx *ctx, double *rhs, double *max_eig) {
   if (max_eig) {
     *max_eig = mmf::ctrl_stage_max(*ctx->hctrl, 3);
   }
+  return MMF_OK;
+}
+
+// euler::computeRHS with the caller's solved flags and order (euler.hpp:41-43
+// takes both on every call): flags and order change only where they differ
+// from the context's.
+int mmf_compute_rhs_ex(mmf_ctx *ctx, const uint8_t *solved, int32_t order, const double *U, double *rhs,
+                       double *max_eig) {
+  if (!ctx) return MMF_ERR_INVALID;
+  if (order != 1 && order != 2) return fail(ctx, MMF_ERR_UNSUPPORTED_ORDER, "reconstruction order (reconstruction.cpp:75-76)");
+  if (order != ctx->g.order) {
+    if (ctx->learn_on) return fail(ctx, MMF_ERR_STATE, "order: the learned reconstruction is on");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->g.order = order;
+    ctx->desc.order = order;
+    ctx->lam_valid = false;
+    ctx->tiles_valid = false;
+  }
+  if (solved) {
+    const int64_t N = ctx->g.N;
+    std::vector<uint8_t> fluid((size_t)N), want((size_t)N);
+    bool same = true;
+    for (int64_t r = 0; r < N; ++r) {
+      const int64_t c = ctx->raw.empty() ? r : ctx->raw[(size_t)r];
+      const uint8_t cur = ctx->hflags.empty() ? (uint8_t)(MMF_CELL_FLUID | MMF_CELL_SOLVED) : ctx->hflags[(size_t)c];
+      fluid[(size_t)r] = (cur & MMF_CELL_FLUID) ? 1 : 0;
+      want[(size_t)r] = solved[r] ? 1 : 0;
+      same = same && (((cur & MMF_CELL_SOLVED) ? 1 : 0) == want[(size_t)r]);
+    }
+    if (!same) {
+      const int rc = mmf_set_cell_flags(ctx, fluid.data(), want.data());
+      if (rc != MMF_OK) return rc;
+    }
+  }
+  if (U) {
+    const int rc = mmf_set_state(ctx, U);
+    if (rc != MMF_OK) return rc;
+  }
+  return mmf_compute_rhs(ctx, rhs, max_eig);
+}
+
+// ---- one RK3 stage at a time (main.cpp:377-506) ----------------------------
+
+namespace {
+double *state_ptr(mmf_ctx *ctx, int32_t state) {
+  return state == MMF_STATE_U ? ctx->dU : state == MMF_STATE_W ? ctx->dW : state == MMF_STATE_RHS ? ctx->dR : nullptr;
+}
+}  // namespace
+
+// reconstruction::computePolynomials + euler::computeRHS of U or W into R
+// (main.cpp:388-389, 432-433, 468-469); the stage maximum of the faces.
+int mmf_rk_rhs(mmf_ctx *ctx, int32_t state, double *max_eig) {
+  if (!ctx || (state != MMF_STATE_U && state != MMF_STATE_W)) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
+  int rc = ensure_tiles(ctx);
+  if (rc != MMF_OK) return rc;
+  hipError_t e = mmf::launch_zero_slot(ctx->dctrl, 3, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "zero slot");
+  rc = launch_stage_on(ctx, 1, mmf::MODE_RHS_ONLY, state_ptr(ctx, state), ctx->dR, ctx->dW);
+  if (rc != MMF_OK) return rc;
+  rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  const double m = mmf::ctrl_stage_max(*ctx->hctrl, 3);
+  ctx->rk_eig[ctx->rk_done % 3] = m;
+  if (max_eig) *max_eig = m;
+  return MMF_OK;
+}
+
+// main.cpp:398-402 for the step's (possibly reduced over ranks) maximum.
+int mmf_rk_dt(mmf_ctx *ctx, double max_eig, double *dt) {
+  if (!ctx || !dt) return MMF_ERR_INVALID;
+  if (!(max_eig > 0.0) || !std::isfinite(max_eig))
+    return fail(ctx, MMF_ERR_STATE, "the step's maximum eigenvalue is not finite and positive");
+  int rc = read_ctrl(ctx);
+  if (rc != MMF_OK) return rc;
+  const Ctrl &c = *ctx->hctrl;
+  double d = 0.9 * c.cfl * c.hmin / max_eig;
+  if (c.t + d > c.tmax) d = c.tmax - c.t;
+  *dt = d;
+  return MMF_OK;
+}
+
+// The update loop of stage 1 (main.cpp:409-422), 2 (445-458) or 3 (481-494);
+// after stage 3, t += dt and the step is counted (main.cpp:505-506).
+int mmf_rk_update(mmf_ctx *ctx, int32_t stage, double dt) {
+  if (!ctx || stage < 1 || stage > 3 || !std::isfinite(dt)) return MMF_ERR_INVALID;
+  ctx->lam_valid = false;
+  hipError_t e = mmf::launch_rk_update(ctx->g, stage, ctx->dU, ctx->dR, ctx->dW, dt, ctx->stream);
+  if (e == hipSuccess && stage == 3) e = mmf::launch_rk_close(ctx->dctrl, dt, ctx->rk_eig, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "stage update");
+  ctx->rk_done = stage % 3;
+  return MMF_OK;
+}
+
+// Ghost lists: rows (host order, as mmf_set_state) of a state to or from a
+// host AoS buffer [n][5] (communications.tpp:164-199).
+static int cells_transfer(mmf_ctx *ctx, int32_t state, const int64_t *cells, int64_t n, double *buf, bool pack) {
+  double *X = ctx ? state_ptr(ctx, state) : nullptr;
+  if (!X || n < 0 || (n && (!cells || !buf))) return ctx ? fail(ctx, MMF_ERR_INVALID, "cell list") : MMF_ERR_INVALID;
+  if (n == 0) return MMF_OK;
+  std::vector<int64_t> c((size_t)n);
+  for (int64_t j = 0; j < n; ++j) {
+    if (cells[j] < 0 || cells[j] >= ctx->g.N) return fail(ctx, MMF_ERR_INVALID, "cell list: index out of range");
+    c[(size_t)j] = ctx->raw.empty() ? cells[j] : ctx->raw[(size_t)cells[j]];
+  }
+  const size_t need = (size_t)n * (5 * sizeof(double) + sizeof(int64_t));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->dpack_bytes < need) {
+    if (ctx->dpack) (void)hipFree(ctx->dpack);
+    ctx->dpack = nullptr;
+    ctx->dpack_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->dpack, need));
+    ctx->dpack_bytes = need;
+  }
+  double *dbuf = (double *)ctx->dpack;
+  int64_t *dcells = (int64_t *)(dbuf + 5 * n);
+  HIPCHK(ctx, hipMemcpyAsync(dcells, c.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  if (pack) {
+    HIPCHK(ctx, mmf::launch_pack_cells(X, ctx->g.fs, dcells, n, dbuf, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(buf, dbuf, sizeof(double) * 5 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    HIPCHK(ctx, hipMemcpyAsync(dbuf, buf, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, mmf::launch_unpack_cells(X, ctx->g.fs, dcells, n, dbuf, ctx->stream));
+    ctx->lam_valid = false;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return MMF_OK;
+}
+
+int mmf_pack_cells(mmf_ctx *ctx, int32_t state, const int64_t *cells, int64_t n, double *buf) {
+  return cells_transfer(ctx, state, cells, n, buf, true);
+}
+
+int mmf_unpack_cells(mmf_ctx *ctx, int32_t state, const int64_t *cells, int64_t n, const double *buf) {
+  return cells_transfer(ctx, state, cells, n, const_cast<double *>(buf), false);
+}
+
+int mmf_get_device_state(mmf_ctx *ctx, int32_t state, double **ptr, int64_t *field_stride) {
+  if (!ctx || !ptr || !state_ptr(ctx, state)) return MMF_ERR_INVALID;
+  *ptr = state_ptr(ctx, state);
+  if (field_stride) *field_stride = ctx->g.fs;
   return MMF_OK;
 }
 

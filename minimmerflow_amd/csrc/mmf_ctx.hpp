@@ -62,6 +62,13 @@ struct mmf_ctx {
   uint8_t *dbslot = nullptr;
   int32_t *dbslot_bc = nullptr;
   double *dbslot_info = nullptr;
+  // one stage at a time (mmf_rk_*): the stage maxima of the current step
+  // and the stages updated so far (0-2)
+  double rk_eig[3] = {0.0, 0.0, 0.0};
+  int rk_done = 0;
+  // ghost-list staging (mmf_pack_cells / mmf_unpack_cells)
+  void *dpack = nullptr;
+  size_t dpack_bytes = 0;
   double *dFS = nullptr;
   unsigned long long *dlcount = nullptr;
   mmf_output *out = nullptr;

@@ -167,6 +167,16 @@ int64_t error_norm_blocks(int64_t N);
 hipError_t launch_mark_bodies(const GridDev &g, const double *origin, const double *boxes, int nboxes,
                               const double *spheres, int nspheres, uint8_t *flags, hipStream_t stream);
 hipError_t launch_fill_double(double *p, int64_t n, double v, hipStream_t stream);
+// One RK3 stage's update (stage 1: W = U + dt R / V; 2: W = 3/4 U + 1/4 (W +
+// dt R / V); 3: U = 1/3 U + 2/3 (W + dt R / V)) and the step's close.
+hipError_t launch_rk_update(const GridDev &g, int stage, double *U, const double *R, double *W, double dt,
+                            hipStream_t stream);
+hipError_t launch_rk_close(Ctrl *ctrl, double dt, const double *eig, hipStream_t stream);
+// Rows of a SoA state at a list of cells (cell order) to / from AoS [n][5].
+hipError_t launch_pack_cells(const double *X, int64_t fs, const int64_t *cells, int64_t n, double *buf,
+                             hipStream_t stream);
+hipError_t launch_unpack_cells(double *X, int64_t fs, const int64_t *cells, int64_t n, const double *buf,
+                               hipStream_t stream);
 
 // Batched point functions.
 hipError_t launch_batch_cons2prim(const double *c, int64_t n, double *p, hipStream_t s);

@@ -488,6 +488,86 @@ hipError_t launch_fill_double(double *p, int64_t n, double v, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// one RK3 stage at a time: the update loops of main.cpp:409-422 (stage 1),
+// 445-458 (stage 2) and 481-494 (stage 3) as their own launch, the same
+// expressions as the fused stage kernels (rk_update, StageIn<2>)
+// ---------------------------------------------------------------------------
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_rk_update(GridDev g, double *U, const double *R, double *W, double dt) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= g.N) return;
+  const bool solved = g.flags == nullptr || (g.flags[i] & CELL_SOLVED);
+  const int64_t fs = g.fs;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int64_t o = k * fs + i;
+    if (STAGE == 1) W[o] = solved ? U[o] + div_vol(g, dt * R[o]) : U[o];
+    else if (STAGE == 2) W[o] = solved ? rk_update<2>(g, U[o], W[o], R[o], dt) : U[o];
+    else if (solved) U[o] = rk_update<3>(g, U[o], W[o], R[o], dt);
+  }
+}
+
+// main.cpp:505-506 after a stage-3 update: t += dt, the step counted, the
+// three stage maxima kept as the step's
+__global__ void k_rk_close(Ctrl *c, double dt, double e0, double e1, double e2) {
+  if (threadIdx.x != 0) return;
+  c->last_eig[0] = e0;
+  c->last_eig[1] = e1;
+  c->last_eig[2] = e2;
+  c->dt = dt;
+  c->t += dt;
+  c->step += 1;
+}
+
+hipError_t launch_rk_update(const GridDev &g, int stage, double *U, const double *R, double *W, double dt,
+                            hipStream_t s) {
+  const dim3 grid((unsigned)((g.N + 255) / 256));
+  if (stage == 1) hipLaunchKernelGGL(k_rk_update<1>, grid, dim3(256), 0, s, g, U, R, W, dt);
+  else if (stage == 2) hipLaunchKernelGGL(k_rk_update<2>, grid, dim3(256), 0, s, g, U, R, W, dt);
+  else hipLaunchKernelGGL(k_rk_update<3>, grid, dim3(256), 0, s, g, U, R, W, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_rk_close(Ctrl *ctrl, double dt, const double *eig, hipStream_t s) {
+  hipLaunchKernelGGL(k_rk_close, dim3(1), dim3(64), 0, s, ctrl, dt, eig[0], eig[1], eig[2]);
+  return hipGetLastError();
+}
+
+// Ghost lists (communications.tpp:164-199 packs 5 doubles per listed cell):
+// rows of the SoA state at cells[0..n) (cell order) <-> AoS buf [n][5].
+__global__ __launch_bounds__(256) void k_pack_cells(const double *X, int64_t fs, const int64_t *cells, int64_t n,
+                                                    double *buf) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t c = cells[j];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) buf[NF * j + k] = X[k * fs + c];
+}
+
+__global__ __launch_bounds__(256) void k_unpack_cells(double *X, int64_t fs, const int64_t *cells, int64_t n,
+                                                      const double *buf) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t c = cells[j];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) X[k * fs + c] = buf[NF * j + k];
+}
+
+hipError_t launch_pack_cells(const double *X, int64_t fs, const int64_t *cells, int64_t n, double *buf,
+                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, fs, cells, n, buf);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_cells(double *X, int64_t fs, const int64_t *cells, int64_t n, const double *buf,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unpack_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, fs, cells, n, buf);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // batched point functions (AoS [count][5] in and out)
 // ---------------------------------------------------------------------------
 __global__ void k_b_c2p(const double *c, int64_t n, double *p) {

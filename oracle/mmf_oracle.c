@@ -1024,6 +1024,16 @@ void orc_solver_cell_centroids(const orc_solver *s, double *out)
 {
     memcpy(out, s->mesh.cell_centroid, sizeof(double) * 3 * s->mesh.ncells);
 }
+/* main.cpp:231-235: cells solved by another rank (ghosts) stay fluid but are
+ * not solved here.  A solved cell must be fluid; returns -1 otherwise. */
+int orc_solver_set_solved(orc_solver *s, const uint8_t *solved)
+{
+    for (int64_t c = 0; c < s->mesh.ncells; ++c)
+        if (solved[c] && !s->fluid[c]) return -1;
+    for (int64_t c = 0; c < s->mesh.ncells; ++c) s->solved[c] = solved[c] ? 1 : 0;
+    return 0;
+}
+
 void orc_solver_flags(const orc_solver *s, uint8_t *fluid, uint8_t *solved)
 {
     memcpy(fluid, s->fluid, s->mesh.ncells);

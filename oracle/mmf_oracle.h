@@ -193,6 +193,7 @@ int64_t orc_solver_ncells(const orc_solver *s);
 int64_t orc_solver_nfaces(const orc_solver *s);
 void orc_solver_counters(const orc_solver *s, int64_t *neg_p, int64_t *neg_rho);
 void orc_solver_cell_centroids(const orc_solver *s, double *out);
+int orc_solver_set_solved(orc_solver *s, const uint8_t *solved);
 void orc_solver_flags(const orc_solver *s, uint8_t *fluid, uint8_t *solved);
 void orc_solver_face_bc(const orc_solver *s, int *out);
 void orc_solver_face_list(const orc_solver *s, int64_t *owner, int64_t *neigh, double *normal, double *area);

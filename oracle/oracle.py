@@ -104,6 +104,8 @@ def _load(path):
     L.orc_solver_counters.argtypes = [C.c_void_p, i64p, i64p]
     L.orc_solver_cell_centroids.argtypes = [C.c_void_p, dp]
     L.orc_solver_flags.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
+    L.orc_solver_set_solved.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+    L.orc_solver_set_solved.restype = C.c_int
     L.orc_solver_face_bc.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.orc_initial_state.argtypes = [C.c_void_p, dp]
     L.orc_compute_polynomials.argtypes = [C.c_void_p, dp]
@@ -300,6 +302,12 @@ class Solver:
         lib().orc_solver_flags(self.s, f.ctypes.data_as(C.POINTER(C.c_uint8)),
                                s.ctypes.data_as(C.POINTER(C.c_uint8)))
         return f, s
+
+    def set_solved(self, solved):
+        """Cells solved here (main.cpp:231-235: ghosts fluid but unsolved)."""
+        sv = np.ascontiguousarray(solved, dtype=np.uint8)
+        if lib().orc_solver_set_solved(self.s, sv.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+            raise ValueError("a solved cell must be fluid")
 
     def face_bc(self):
         out = np.empty(self.nfaces, dtype=np.int32)

@@ -304,3 +304,80 @@ def test_threaded_steps_equal_serial(orc, name, dim, n, bc, boxes, order):
         got = out[threads]
         assert np.array_equal(got[0], ref[0]) and got[1] == ref[1] and got[2] == ref[2]
         assert got[3] == ref[3] and np.array_equal(got[4], ref[4])
+
+
+def _partitioned_steps(orc, order, G, nsteps=3):
+    """Two sub-grids of a 20 x 16 grid split at row 8, each holding G rows of
+    the other's cells as ghosts, stepped stage by stage with the ghosts
+    exchanged after every stage and the eigenvalue reduced over both; returns
+    the sub-grids' own rows joined and the whole grid's state."""
+    n, cut, h, cfl = [20, 16], 8, 0.25, 0.45
+    side_bc = [1, 0, 3, 1, 0, 0]
+    info = np.tile([1.0, 0.2, 0.4, 0.0, 1.0 / 1.4], (6, 1))
+    pb = orc.problem(orc.RADSOD, 2)
+    rng = np.random.default_rng(order)
+    N = n[0] * n[1]
+    U0 = np.array([orc.prim2cons([rng.uniform(0.5, 2), *rng.uniform(-0.6, 0.6, 2), 0.0, rng.uniform(0.5, 2)])
+                   for _ in range(N)])
+    cx, cy = (np.arange(N) % n[0] + 0.5) * h, (np.arange(N) // n[0] + 0.5) * h
+    fluid = (~((cx > 1.5) & (cx < 2.6) & (cy > 1.4) & (cy < 2.6))).astype(np.uint8)  # a body across the cut
+    whole = orc.Solver(pb, order, n, [0.0] * 3, h, side_bc=side_bc, side_info=info, fluid=fluid)
+    parts = []
+    for k, (lo, hi) in enumerate([(0, cut + G), (cut - G, n[1])]):
+        sl = slice(lo * n[0], hi * n[0])
+        s = orc.Solver(pb, order, [n[0], hi - lo], [0.0, lo * h, 0.0], h, side_bc=side_bc, side_info=info,
+                       fluid=fluid[sl])
+        rows = hi - lo
+        ghost = np.arange((rows - G) * n[0], rows * n[0]) if k == 0 else np.arange(0, G * n[0])
+        send = np.arange((rows - 2 * G) * n[0], (rows - G) * n[0]) if k == 0 else np.arange(G * n[0], 2 * G * n[0])
+        solved = fluid[sl].copy()
+        solved[ghost] = 0
+        s.set_solved(solved)
+        parts.append({"s": s, "U": U0[sl].copy(), "W": U0[sl].copy(), "solved": solved.astype(bool),
+                      "ghost": ghost, "send": send})
+    a, b = parts
+
+    def exchange(key):
+        va, vb = a[key][a["send"]].copy(), b[key][b["send"]].copy()
+        b[key][b["ghost"]] = va
+        a[key][a["ghost"]] = vb
+
+    Uw, Ww = U0.copy(), U0.copy()
+    t = 0.0
+    V = h * h
+    dts = []
+    for _ in range(nsteps):
+        R = [p["s"].compute_rhs(p["U"]) for p in parts]
+        dt = 0.9 * cfl * h / max(R[0][1], R[1][1])  # main.cpp:390-402
+        for p, (Rp, _) in zip(parts, R):
+            m = p["solved"]
+            p["W"][m] = p["U"][m] + dt * Rp[m] / V  # main.cpp:409-422
+        exchange("W")
+        for stage in (2, 3):
+            R = [p["s"].compute_rhs(p["W"]) for p in parts]
+            for p, (Rp, _) in zip(parts, R):
+                m = p["solved"]
+                if stage == 2:  # main.cpp:445-458
+                    p["W"][m] = 0.75 * p["U"][m] + 0.25 * (p["W"][m] + dt * Rp[m] / V)
+                else:  # main.cpp:481-494
+                    p["U"][m] = (1. / 3) * p["U"][m] + (2. / 3) * (p["W"][m] + dt * Rp[m] / V)
+            exchange("W" if stage == 2 else "U")
+        dtw, _ = whole.rk3_step(Uw, Ww, cfl, t, 1e30)
+        dts.append((dt, dtw))
+        t += dtw
+    return np.concatenate([a["U"][:cut * n[0]], b["U"][G * n[0]:]]), Uw, dts
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_partitioned_oracle_equals_the_whole_grid(orc, order):
+    """Domain decomposition (main.cpp:222-237, 377-506 under MPI): ghosts as
+    deep as the stencil give the whole grid's steps bit for bit.  Pins the
+    oracle's unsolved fluid cells: faces between two of them skipped, the
+    cells not updated, their values still read by the reconstruction.  One
+    ghost row under the order-2 stencil (which reaches two cells) is seen."""
+    parts, whole, dts = _partitioned_steps(orc, order, G=order)
+    assert all(d == w for d, w in dts)
+    assert np.array_equal(parts, whole)
+    if order == 2:
+        thin, whole, _ = _partitioned_steps(orc, 2, G=1)
+        assert not np.array_equal(thin, whole)
