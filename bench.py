@@ -204,8 +204,9 @@ def cpu_baseline_batch(descs, problems, budget_s=10.0, budget_mt=5.0):
         cases = []
         for d, pm in zip(descs, problems):
             po = O.problem(keys[pm.type], 2, vortex_beta=pm.vortex_beta, vortex_center=list(pm.vortex_center),
-                           sod_p_in=pm.sod_p_in, sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out,
-                           sod_t_out=pm.sod_t_out)
+                           vortex_periodic=pm.vortex_periodic, domain_origin=list(pm.domain_origin),
+                           domain_length=pm.domain_length, sod_r0=pm.sod_r0, sod_p_in=pm.sod_p_in,
+                           sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out, sod_t_out=pm.sod_t_out)
             n = [int(d.n[0]), int(d.n[1])]
             sv = O.Solver(po, 1, n, list(d.origin), d.h, side_bc=[d.side_bc[k] for k in range(6)])
             U = sv.initial_state()
@@ -293,9 +294,9 @@ def run_batch(args, mmf):
         cpu = cpu_baseline_batch(descs, problems)
         log("cpu baseline: %.3e cell-updates/s" % cpu["value"])
     config = {"workload": workload, "cases": b.ncases, "cells": N, "work_items": b.nitems,
-              "case_set": "SplitMix64 seed 0x6D6D66: side ~ {32,48,...,256}; even cases isentropic vortex "
-                          "(beta~U[2,6], centre~U[-2,2]^2, [-5,5]^2, free flow), odd cases radial Sod "
-                          "(rho and p ratios ~U[4,16], [0,8]^2, r0=1, reflecting)",
+              "case_set": "SplitMix64 seed 0x6D6D66: side ~ {32,48,...,256}; cases 0-255 isentropic vortex "
+                          "(beta~U[2,6], centre~U[-2,2]^2, [-5,5]^2, periodic), cases 256-511 radial Sod "
+                          "(rho and p ratios ~U[4,16], [-1,1]^2, r0=0.4, transmissive)",
               "sides": WL.side_histogram(params), "time": "SSP-RK3, CFL 0.5, fixed step count, per-case dt",
               "shape": {"rows_per_item": chunk, "persistent_blocks": blocks, "autotuned": tuned},
               "prewarm_steps": nprewarm,

@@ -145,9 +145,9 @@ def test_radsod_256_mass_and_symmetry(mmf):
 
 
 def test_batch_512_mixed_set_steps_bitwise(mmf, orc):
-    """The 512-case mixed set (13.2 M cells: vortices on free-flow grids and
-    radial Sod on reflecting ones, sides 32..256) at the autotuned launch
-    shape: two steps of every case equal the oracle's bit for bit."""
+    """The 512-case mixed set (periodic vortices and transmissive radial Sod,
+    sides 32..256) at the autotuned launch shape: two steps of every case
+    equal the oracle's bit for bit."""
     from minimmerflow_amd import workloads as WL
     descs, problems, _ = WL.mixed_2d_cases(mmf, 512, WL.DEFAULT_SEED)
     b = mmf.Batch(descs)
@@ -161,7 +161,9 @@ def test_batch_512_mixed_set_steps_bitwise(mmf, orc):
     same = 0
     for i, (d, pm) in enumerate(zip(descs, problems)):
         po = orc.problem(keys[pm.type], 2, vortex_beta=pm.vortex_beta, vortex_center=list(pm.vortex_center),
-                         sod_p_in=pm.sod_p_in, sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out, sod_t_out=pm.sod_t_out)
+                         vortex_periodic=pm.vortex_periodic, domain_origin=list(pm.domain_origin),
+                         domain_length=pm.domain_length, sod_r0=pm.sod_r0, sod_p_in=pm.sod_p_in,
+                         sod_t_in=pm.sod_t_in, sod_p_out=pm.sod_p_out, sod_t_out=pm.sod_t_out)
         n = [int(d.n[0]), int(d.n[1])]
         o = orc.Solver(po, 1, n, list(d.origin), d.h, side_bc=[d.side_bc[k] for k in range(6)])
         U = o.initial_state()
