@@ -262,13 +262,16 @@ def run_batch(args, mmf):
     N = b.total_cells
     log("setup %.1fs: %d cases, %d cells, %d work items" % (time.perf_counter() - t_setup, b.ncases, N, b.nitems))
     if args.chunk or args.blocks:
+        b.set_tail(args.tail)
         b.set_shape(args.chunk, args.blocks)
         tuned = None
     else:
         chunk, blocks, ms = b.autotune(steps_per_trial=3)
-        tuned = {"chunk": chunk, "blocks": blocks, "ms_per_step": ms}
-        log("autotune: %d rows per item, %d persistent blocks (%.4f ms/step)" % (chunk, blocks, ms))
+        tuned = {"chunk": chunk, "tail": b.tail(), "blocks": blocks, "ms_per_step": ms}
+        log("autotune: %d rows per item, tail items of %d rows, %d persistent blocks (%.4f ms/step)"
+            % (chunk, b.tail(), blocks, ms))
     chunk, blocks = b.shape()
+    w.set_tail(b.tail())
     w.set_shape(chunk, blocks)
     nprewarm = prewarm(w.advance, w.times)
     b.advance(args.warmup)
@@ -298,7 +301,8 @@ def run_batch(args, mmf):
                           "(beta~U[2,6], centre~U[-2,2]^2, [-5,5]^2, periodic), cases 256-511 radial Sod "
                           "(rho and p ratios ~U[4,16], [-1,1]^2, r0=0.4, transmissive)",
               "sides": WL.side_histogram(params), "time": "SSP-RK3, CFL 0.5, fixed step count, per-case dt",
-              "shape": {"rows_per_item": chunk, "persistent_blocks": blocks, "autotuned": tuned},
+              "shape": {"rows_per_item": chunk, "tail_rows": b.tail(), "persistent_blocks": blocks,
+                        "autotuned": tuned},
               "prewarm_steps": nprewarm,
               "reconstruction": "order 1", "flux": "Rusanov", "parallelism": "single GPU"}
     out = {
@@ -599,6 +603,8 @@ def main():
     ap.add_argument("--cols", type=int, default=0, help="marching kernel: tile columns (64 or 32)")
     ap.add_argument("--chunk", type=int, default=0, help="marching kernel: planes per block")
     ap.add_argument("--blocks", type=int, default=0, help="batch: persistent blocks (default: autotuned)")
+    ap.add_argument("--tail", type=int, default=0, help="batch: rows of the small items ending each stage "
+                                                         "(with --chunk/--blocks; 0: none)")
     ap.add_argument("--ncases", type=int, default=512, help="batch: number of cases")
     ap.add_argument("--train-cases", type=int, default=128, help="learned: fine training runs")
     ap.add_argument("--eval-cases", type=int, default=4, help="learned: held-out cases")

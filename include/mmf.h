@@ -364,6 +364,11 @@ int mmf_batch_set_time(mmf_batch *b, const double *t0, const double *tmax, const
 /* rows per work item and persistent grid size (<= 0 keeps the current value) */
 int mmf_batch_set_shape(mmf_batch *b, int32_t chunk, int32_t blocks);
 int mmf_batch_get_shape(const mmf_batch *b, int32_t *chunk, int32_t *blocks);
+/* rows of the small items that end each stage: the last 'chunk' rows of
+ * every strip split into items of this many rows, queued after the full
+ * items (0: none).  Results do not depend on it. */
+int mmf_batch_set_tail(mmf_batch *b, int32_t rows);
+int mmf_batch_get_tail(const mmf_batch *b, int32_t *rows);
 /* enqueue nsteps RK3 steps (graph replays); cases past their end time idle */
 int mmf_batch_advance(mmf_batch *b, int64_t nsteps);
 /* steps until every case reaches its end time (max_steps < 0: no limit) */
@@ -377,7 +382,7 @@ int mmf_batch_error_vs_exact(mmf_batch *b, const mmf_problem *problems, double *
 /* graph-replay timing with HIP events (whole step: three stages + update) */
 int mmf_batch_enable_timing(mmf_batch *b, int32_t on);
 int mmf_batch_read_timing(mmf_batch *b, double *ms, int64_t *steps);
-/* Launch-shape tuner: times candidate (chunk, blocks) pairs over
+/* Launch-shape tuner: times candidate (chunk, tail, blocks) triples over
  * steps_per_trial steps each on a snapshot of the state, restores the state
  * and keeps the fastest.  Results do not depend on the shape. */
 int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chunk, int32_t *best_blocks,

@@ -160,6 +160,8 @@ def lib():
         "mmf_batch_set_initial_states": ([vp, C.POINTER(Problem)], C.c_int),
         "mmf_batch_set_time": ([vp, dp, dp, dp], C.c_int),
         "mmf_batch_set_shape": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "mmf_batch_set_tail": ([vp, C.c_int32], C.c_int),
+        "mmf_batch_get_tail": ([vp, i32p], C.c_int),
         "mmf_batch_get_shape": ([vp, i32p, i32p], C.c_int),
         "mmf_batch_advance": ([vp, C.c_int64], C.c_int),
         "mmf_batch_run": ([vp, C.c_int64, i64p], C.c_int),
@@ -733,6 +735,16 @@ class Batch:
         a, b = C.c_int32(), C.c_int32()
         self._c(lib().mmf_batch_get_shape(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_tail(self, rows):
+        """Rows of the small items that end each stage (0: none)."""
+        self._c(lib().mmf_batch_set_tail(self.h, int(rows)))
+        self.set_shape(0, 0)
+
+    def tail(self):
+        r = C.c_int32()
+        self._c(lib().mmf_batch_get_tail(self.h, C.byref(r)))
+        return r.value
 
     def advance(self, nsteps):
         self._c(lib().mmf_batch_advance(self.h, int(nsteps)))

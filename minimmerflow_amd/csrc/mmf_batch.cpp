@@ -41,6 +41,7 @@ struct mmf_batch {
   int nitems = 0;
   unsigned int *dcounters = nullptr;
   int chunk = 16;
+  int tail = 0;  // rows of the small items at the end of the list (0: none)
   int blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -73,21 +74,26 @@ void drop_graph(mmf_batch *b) {
   b->graph = nullptr;
 }
 
-// (re)build the work list and the case table for the current chunk
+// (re)build the work list and the case table for the current shape.  An item
+// is (case, strip, first row, end row).  With a tail, the last 'chunk' rows of
+// every strip become items of 'tail' rows, queued after all full items, so
+// each stage drains in small pieces instead of ending on the slowest full
+// item.
 int build_items(mmf_batch *b) {
-  std::vector<int> items;
+  std::vector<int> items, small;
   for (int i = 0; i < b->ncases; ++i) {
     BatchCase &c = b->hcases[(size_t)i];
     c.ntx = mmf::march2d_tiles(c.g.n[0]);
     c.chunk = b->chunk;
-    const int nch = (int)((c.g.n[1] + b->chunk - 1) / b->chunk);
-    for (int z = 0; z < nch; ++z)
-      for (int t = 0; t < c.ntx; ++t) {
-        items.push_back(i);
-        items.push_back(t + c.ntx * z);
-      }
+    const int ny = (int)c.g.n[1];
+    const int split = (b->tail > 0) ? std::max(0, ny - b->chunk) : ny;  // rows of full items
+    for (int m0 = 0; m0 < split; m0 += b->chunk)
+      for (int t = 0; t < c.ntx; ++t) items.insert(items.end(), {i, t, m0, std::min(m0 + b->chunk, split)});
+    for (int m0 = split; m0 < ny; m0 += b->tail)
+      for (int t = 0; t < c.ntx; ++t) small.insert(small.end(), {i, t, m0, std::min(m0 + b->tail, ny)});
   }
-  b->nitems = (int)(items.size() / 2);
+  items.insert(items.end(), small.begin(), small.end());
+  b->nitems = (int)(items.size() / 4);
   if (b->ditems) (void)hipFree(b->ditems);
   b->ditems = nullptr;
   BCHK(b, hipMalloc(&b->ditems, sizeof(int) * items.size()));
@@ -347,6 +353,21 @@ int mmf_batch_set_shape(mmf_batch *b, int32_t chunk, int32_t blocks) {
   return MMF_OK;
 }
 
+int mmf_batch_set_tail(mmf_batch *b, int32_t rows) {
+  if (!b || rows < 0) return MMF_ERR_INVALID;
+  if (rows != b->tail) {
+    b->tail = rows;
+    return build_items(b);
+  }
+  return MMF_OK;
+}
+
+int mmf_batch_get_tail(const mmf_batch *b, int32_t *rows) {
+  if (!b || !rows) return MMF_ERR_INVALID;
+  *rows = b->tail;
+  return MMF_OK;
+}
+
 int mmf_batch_get_shape(const mmf_batch *b, int32_t *chunk, int32_t *blocks) {
   if (!b) return MMF_ERR_INVALID;
   if (chunk) *chunk = b->chunk;
@@ -484,14 +505,18 @@ int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chun
   }
   const int chunks[] = {4, 8, 16, 32, 64};
   const int per_cu[] = {4, 8, 12, 16};
-  const int keep_chunk = b->chunk, keep_blocks = b->blocks;
+  const int keep_chunk = b->chunk, keep_blocks = b->blocks, keep_tail = b->tail;
   const bool keep_timing = b->timing;
-  int bc = keep_chunk, bb = keep_blocks;
+  int bc = keep_chunk, bb = keep_blocks, bt = keep_tail;
   double best = 1e300;
   int rc = MMF_OK;
-  for (int ch : chunks) {
-    for (int pc : per_cu) {
-      rc = mmf_batch_set_shape(b, ch, pc * b->num_cus);
+  for (int trial = 0; trial < 2 * 5 * 4 && rc == MMF_OK; ++trial) {
+    const int ch = chunks[trial / 8], pc = per_cu[trial % 4];
+    const int tl = ((trial / 4) % 2) ? std::max(4, ch / 4) : 0;  // no tail, or items of a quarter
+    if (tl >= ch) continue;
+    {
+      rc = mmf_batch_set_tail(b, tl);
+      if (rc == MMF_OK) rc = mmf_batch_set_shape(b, ch, pc * b->num_cus);
       if (rc == MMF_OK) rc = mmf_batch_enable_timing(b, 1);
       if (rc == MMF_OK) rc = mmf_batch_advance(b, 1);  // warm-up (and graph build)
       if (rc == MMF_OK) rc = mmf_batch_enable_timing(b, 1);
@@ -503,6 +528,7 @@ int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chun
         best = ms / n;
         bc = ch;
         bb = pc * b->num_cus;
+        bt = tl;
       }
       // restore the state for the next trial
       if (hipMemcpyAsync(b->dU, snapU, bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
@@ -510,13 +536,12 @@ int mmf_batch_autotune(mmf_batch *b, int32_t steps_per_trial, int32_t *best_chun
               hipSuccess ||
           hipStreamSynchronize(b->stream) != hipSuccess)
         rc = bfail(b, MMF_ERR_HIP, "autotune restore");
-      if (rc != MMF_OK) break;
     }
-    if (rc != MMF_OK) break;
   }
   (void)hipFree(snapU);
   if (rc != MMF_OK) return rc;
-  rc = mmf_batch_set_shape(b, bc, bb);
+  rc = mmf_batch_set_tail(b, bt);
+  if (rc == MMF_OK) rc = mmf_batch_set_shape(b, bc, bb);
   if (rc == MMF_OK) rc = mmf_batch_enable_timing(b, keep_timing ? 1 : 0);
   if (rc != MMF_OK) return rc;
   if (best_chunk) *best_chunk = bc;

@@ -13,7 +13,7 @@
 // upper, y lower, y upper), bit-identical to the gather kernel and the oracle.
 //
 // The batch kernel is persistent: a fixed grid of one-wave blocks pulls
-// (case, tile, chunk) items from an atomic counter until the list is
+// (case, strip, row range) items from an atomic counter until the list is
 // exhausted; each case has its own grid, time-control block and dt.
 #include <hip/hip_runtime.h>
 
@@ -130,6 +130,7 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
     else halo = false;
   }
   const bool has_tuple = valid || halo;
+  const bool left_halo = x0 > 0 || g.periodic[0];  // lane 0 holds a tuple
   const uint32_t col = (uint32_t)cx;
   const uint32_t sm = (uint32_t)nx;
   const bool has_xm = valid && (gx > 0 || g.periodic[0]);
@@ -256,10 +257,48 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
       ff = cell_flags<MASK>(g, i2);
     }
 
+    // ---- without a mask: each x face once.  Lane k (1 <= k <= w + 1)
+    // evaluates the face between lanes k - 1 and k, the halo lanes and the
+    // lane beyond a border included; a cell takes its upper face from the
+    // next lane.  The face a strip shares with the strip on its left is
+    // evaluated by both, counted by the left one (as the reference counts
+    // each face once). ----
+    double AFx[NF] = {0., 0., 0., 0., 0.};
+    if (!MASK) {
+      if (lane >= 1 && lane <= w + 1) {
+        const bool left_t = lane >= 2 || left_halo;  // lane k - 1 holds a tuple
+        const bool own_t = lane <= w || halo;        // lane k holds one
+        double l;
+        if (left_t && own_t) {
+          Tup Nb;
+          tload(L, lane - 1, Nb);
+          l = face_flux_area(Nb, Tc, 0, P2, A, halfA, AFx);
+          if ((lane >= 2 || !left_halo) && (tup_negative(Nb) | bad_c)) neg.add(Nb, Tc);
+        } else if (own_t) {  // the left border
+          l = vface(g, false, 0, 0, Tc, false, AFx, neg, true);
+        } else {  // the right border: the fluid cell is the lane below
+          Tup Nb;
+          tload(L, lane - 1, Nb);
+          l = vface(g, false, 1, 0, Nb, true, AFx, neg, true);
+        }
+        lam_max = lam_acc(lam_max, l);
+      }
+    }
+    double AFup[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) AFup[k] = MASK ? 0.0 : __shfl_down(AFx[k], 1, W2);
+
     if (valid) {
       // ---- x faces of this cell from the LDS neighbours, reference order ----
       double acc[NF] = {0., 0., 0., 0., 0.};
-      if (fc & CELL_SOLVED) {
+      if (!MASK) {
+        // first two contributions from zero: their order does not change the bits
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          acc[k] += AFx[k];
+          acc[k] -= AFup[k];
+        }
+      } else if (fc & CELL_SOLVED) {
         double AFm[NF], AFp[NF];
         {
           const uint8_t fm = MASK && has_xm ? LF[lane - 1] : (uint8_t)(CELL_FLUID | CELL_SOLVED);
@@ -350,6 +389,213 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
   return lam_max;
 }
 
+// An unmasked item whose x borders are copies (free flow), periodic or absent,
+// with a branch-free row loop: every lane loads (lanes without a cell on a
+// clamped column), builds the next row's tuple and evaluates the x face below
+// it, and results of lanes without a cell are discarded, so the next row's
+// division chain, this row's faces and its update share one basic block and
+// the scheduler interleaves them.  A free-flow border face is the Rusanov
+// flux between the fluid tuple and itself (euler.cpp:298-308).  The same
+// faces, sums and counts as march2d_item<STAGE, false, P2>.
+template <int STAGE, bool P2>
+__device__ double march2d_item_flat(const GridDev &g0, const In2<STAGE, P2> &in, const double *Ub, double *outb,
+                                    double dt, Ctrl *ctrl, int tile, int m0, int m1, double *ldsb) {
+  GridDev g = g0;
+  g.pow2A = P2;
+  g.pow2V = P2;
+  const int lane = threadIdx.x;
+  const int nx = (int)g.n[0];
+  const int nm = (int)g.n[1];
+  const int64_t fs = in.fs;
+  const int x0 = tile * INNER;
+  const int w = (nx - x0 < INNER) ? nx - x0 : INNER;
+  const int gx = x0 - 1 + lane;
+  const bool valid = lane >= 1 && lane <= w;
+  bool halo = (lane == 0) || (lane == w + 1);
+  int cx = gx;
+  if (halo && (cx < 0 || cx >= nx)) {
+    if (g.periodic[0]) cx = (cx + nx) % nx;
+    else halo = false;
+  }
+  if (!valid && !halo) cx = (gx < 0) ? 0 : ((gx >= nx) ? nx - 1 : gx);  // a clamped column, results unused
+  const bool left_halo = x0 > 0 || g.periodic[0];
+  const uint32_t col = (uint32_t)cx;
+  const uint32_t sm = (uint32_t)nx;
+  const bool top_wrap_here = g.periodic[1] && m1 == nm;
+  const bool have_beyond = (m1 < nm) || g.periodic[1];
+  const uint32_t beyond = (m1 < nm) ? (uint32_t)m1 : 0u;
+  const double A = g.A, halfA = g.halfA;
+  const bool face_lane = lane >= 1 && lane <= w + 1;  // evaluates the x face below it
+  const bool left_t = lane >= 2 || left_halo;         // lane - 1 holds a tuple
+  const bool own_t = lane <= w || halo;               // this lane holds one
+  // each face counted once: a strip's first face belongs to the strip on its left
+  const bool count_x = face_lane && (lane >= 2 || !left_halo);
+
+  double lam_max = 0.0;
+  NegCount neg;
+  Tup Tc;
+  bool bad_c;
+  double AFlo[NF], Ucur[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) AFlo[k] = Ucur[k] = 0.0;
+
+  // ---- prologue: row m0 into LDS buffer 0, and the face below it ----
+  {
+    const uint32_t idx0 = col + (uint32_t)m0 * sm;
+    Pf2<STAGE> p0;
+    double X[NF];
+    p0.issue(in, idx0);
+    p0.state(g, dt, X);
+    if (STAGE == 2) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) Ucur[k] = p0.a[k];
+    }
+    make_tup(X, Tc);
+    bad_c = tup_negative(Tc);
+    tstore(ldsb, lane, Tc);
+    if (valid) {
+      int mprev = m0 - 1;
+      bool have_prev = true;
+      if (mprev < 0) {
+        if (g.periodic[1]) mprev += nm;
+        else have_prev = false;
+      }
+      double l;
+      if (!have_prev) {  // lower border
+        l = vface(g, false, 2, 1, Tc, false, AFlo, neg, true);
+      } else {
+        double Xp[NF];
+        Tup Tp;
+        in.state(g, dt, col + (uint32_t)mprev * sm, Xp);
+        make_tup(Xp, Tp);
+        // counted here: the item below does not count its chunk's top face
+        l = face_flux_area(Tp, Tc, 1, P2, A, halfA, AFlo);
+        if (tup_negative(Tp) | bad_c) neg.add(Tp, Tc);
+      }
+      lam_max = lam_acc(lam_max, l);
+    }
+  }
+
+  // loads of row m0 + 1 (every lane)
+  Pf2<STAGE> pf;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) pf.a[k] = pf.b[k] = 0.0;
+  if (m0 + 1 < m1 || have_beyond) pf.issue(in, col + ((m0 + 1 < m1) ? (uint32_t)(m0 + 1) : beyond) * sm);
+  __syncthreads();
+
+  int buf = 0;
+  for (int m = m0; m < m1; ++m) {
+    const uint32_t idx = col + (uint32_t)m * sm;
+    const double *L = ldsb + buf * (NTUP * W2);
+    double *Ln = ldsb + (buf ^ 1) * (NTUP * W2);
+    const bool in_chunk = m + 1 < m1;                // row m + 1 is this chunk's
+    const bool has_next = in_chunk || have_beyond;  // a cell above this one
+    if (STAGE == 3) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) Ucur[k] = ld(Ub, fs, k, idx);
+    }
+    // ---- row m + 1: tuples into the other LDS buffer ----
+    Tup Tn;
+    bool bad_n = false;
+    double Unext[NF];
+    if (has_next) {
+      double X[NF];
+      pf.state(g, dt, X);
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Unext[k] = pf.a[k];
+      }
+      make_tup(X, Tn);
+      bad_n = tup_negative(Tn);
+      if (in_chunk) tstore(Ln, lane, Tn);
+    }
+    // the loads of row m + 2
+    if (m + 2 < m1 || (m + 2 == m1 && have_beyond))
+      pf.issue(in, col + ((m + 2 < m1) ? (uint32_t)(m + 2) : beyond) * sm);
+
+    // ---- the x face below this lane; a border's copy state is the fluid tuple ----
+    double AFx[NF];
+    {
+      Tup Nb;
+      tload(L, lane > 0 ? lane - 1 : 0, Nb);
+      Tup Lt, Rt;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        Lt.c[k] = left_t ? Nb.c[k] : Tc.c[k];
+        Rt.c[k] = own_t ? Tc.c[k] : Nb.c[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        Lt.vel[k] = left_t ? Nb.vel[k] : Tc.vel[k];
+        Rt.vel[k] = own_t ? Tc.vel[k] : Nb.vel[k];
+      }
+      Lt.p = left_t ? Nb.p : Tc.p;
+      Rt.p = own_t ? Tc.p : Nb.p;
+      Lt.a = left_t ? Nb.a : Tc.a;
+      Rt.a = own_t ? Tc.a : Nb.a;
+      Lt.H = left_t ? Nb.H : Tc.H;
+      Rt.H = own_t ? Tc.H : Nb.H;
+      const double l = face_flux_area(Lt, Rt, 0, P2, A, halfA, AFx);
+      lam_max = lam_acc(lam_max, face_lane ? l : 0.0);
+      if (count_x) neg.add(Lt, Rt);
+    }
+    double AFup[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) AFup[k] = __shfl_down(AFx[k], 1, W2);
+
+    // ---- the y face above this cell (each y face once) ----
+    double AFhi[NF];
+    if (has_next) {
+      const double l = face_flux_area(Tc, Tn, 1, P2, A, halfA, AFhi);
+      lam_max = lam_acc(lam_max, valid ? l : 0.0);
+      // the chunk's top face and the periodic wrap face are counted by the item above
+      if (valid && in_chunk && (bad_c | bad_n)) neg.add(Tc, Tn);
+    } else {  // upper border
+      const double l = vface(g, false, 3, 1, Tc, true, AFhi, neg, valid);
+      lam_max = lam_acc(lam_max, valid ? l : 0.0);
+    }
+
+    // ---- this cell is complete: its RK stage update ----
+    const bool wrap_top = top_wrap_here && m == nm - 1;
+    double nu[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      double acc = 0.0;  // first two contributions from zero: their order does not change the bits
+      acc += AFx[k];
+      acc -= AFup[k];
+      const double rhs = wrap_top ? (acc - AFhi[k]) + AFlo[k] : (acc + AFlo[k]) - AFhi[k];
+      nu[k] = rk_update<STAGE>(g, Ucur[k], Tc.c[k], rhs, dt);
+    }
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) outb[k * fs + idx] = nu[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
+    if (has_next) {
+      Tc = Tn;
+      bad_c = bad_n;
+      if (STAGE == 2) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) Ucur[k] = Unext[k];
+      }
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  neg.flush(ctrl, lane);
+  return lam_max;
+}
+
+// The flat item applies: the strip's x borders are copies, periodic or absent.
+__device__ __forceinline__ bool flat_item(const GridDev &g, int tile) {
+  const int nx = (int)g.n[0];
+  const int x0 = tile * INNER;
+  const bool lo = x0 > 0 || g.periodic[0] || g.bc[0] == BC_FREE_FLOW;
+  const bool hi = x0 + INNER < nx || g.periodic[0] || g.bc[1] == BC_FREE_FLOW;
+  return lo && hi;
+}
+
 // wave-wide max of non-negative doubles as u64 bits; lane 0 holds the result
 __device__ __forceinline__ unsigned long long wave_max_bits(double v) {
   unsigned long long b = dbits(v);
@@ -377,7 +623,9 @@ __global__ __launch_bounds__(W2) void k_march2d(GridDev g, const double *X, cons
   const int m0 = bz * chunk;
   const int m1 = (m0 + chunk < nm) ? m0 + chunk : nm;
   In2<STAGE, P2> in{X, Uin, R, g.fs};
-  double lam = march2d_item<STAGE, MASK, P2>(g, in, Uin, out, dt, ctrl, tile, m0, m1, lds, lfl);
+  const double lam = (!MASK && flat_item(g, tile))
+                         ? march2d_item_flat<STAGE, P2>(g, in, Uin, out, dt, ctrl, tile, m0, m1, lds)
+                         : march2d_item<STAGE, MASK, P2>(g, in, Uin, out, dt, ctrl, tile, m0, m1, lds, lfl);
   unsigned long long b = wave_max_bits(lam);
   if (threadIdx.x == 0) stage_max_put(ctrl, (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1, b);
 }
@@ -732,27 +980,28 @@ __global__ __launch_bounds__(W2, (STAGE == 1) ? 3 : 2) void k_batch2d(const Batc
     const int it = item_s;
     __syncthreads();
     if (it >= nitems) break;  // every wave of the block reaches this exit
-    const int ci = items[2 * it];
-    const int code = items[2 * it + 1];
+    // item: case, strip, rows [m0, m1)
+    const int ci = items[4 * it], tile = items[4 * it + 1], m0 = items[4 * it + 2], m1 = items[4 * it + 3];
     const BatchCase &bc = cases[ci];
     Ctrl *ctrl = bc.ctrl;
     if (!(ctrl->t < ctrl->tmax)) continue;  // this case has reached its end time
     const GridDev &g = bc.g;
     const double dt = (STAGE >= 2) ? step_dt_line0(ctrl) : 0.0;
-    const int tile = code % bc.ntx;
-    const int bz = code / bc.ntx;
-    const int nm = (int)g.n[1];
-    const int m0 = bz * bc.chunk;
-    const int m1 = (m0 + bc.chunk < nm) ? m0 + bc.chunk : nm;
     double *Ub = U + bc.offset, *Rb = R + bc.offset, *Wb = W + bc.offset;
     const double *X = (STAGE == 3) ? Wb : Ub;
     double *out = (STAGE == 1) ? Rb : (STAGE == 2) ? Wb : Ub;
     // batch cases carry no body mask (all cells fluid and solved)
     double lam;
+    const bool flat = flat_item(g, tile);
     if (g.pow2A && g.pow2V)
-      lam = march2d_item<STAGE, false, true>(g, In2<STAGE, true>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds, lfl);
+      lam = flat ? march2d_item_flat<STAGE, true>(g, In2<STAGE, true>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds)
+                 : march2d_item<STAGE, false, true>(g, In2<STAGE, true>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1,
+                                                    lds, lfl);
     else
-      lam = march2d_item<STAGE, false, false>(g, In2<STAGE, false>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1, lds, lfl);
+      lam = flat ? march2d_item_flat<STAGE, false>(g, In2<STAGE, false>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0, m1,
+                                                   lds)
+                 : march2d_item<STAGE, false, false>(g, In2<STAGE, false>{X, Ub, Rb, g.fs}, Ub, out, dt, ctrl, tile, m0,
+                                                     m1, lds, lfl);
     unsigned long long b = wave_max_bits(lam);
     if (threadIdx.x == 0 && b) atomicMax(&ctrl->eig[STAGE - 1][0][0], b);
   }

@@ -99,12 +99,16 @@ def test_batch_equals_single_grid(mmf):
     assert np.array_equal(b.get_state(6), states[6])
 
 
-@pytest.mark.parametrize("chunk,blocks", [(1, 1), (3, 7), (64, 2048), (1000, 300)])
-def test_batch_shape_independent(mmf, chunk, blocks):
+@pytest.mark.parametrize("chunk,blocks,tail", [(1, 1, 0), (3, 7, 0), (64, 2048, 0), (1000, 300, 0), (16, 500, 4),
+                                               (64, 64, 5), (8, 3, 1)])
+def test_batch_shape_independent(mmf, chunk, blocks, tail):
     cs, states = setup(mmf)
     ref = make_batch(mmf, cs, states)
     ref.advance(STEPS)
     b = make_batch(mmf, cs, states, chunk, blocks)
+    if tail:
+        b.set_tail(tail)
+        assert b.tail() == tail
     assert b.shape() == (chunk, blocks)
     b.advance(STEPS)
     assert np.array_equal(b.times()[0], ref.times()[0])
