@@ -16,7 +16,7 @@ LIB = os.path.join(LIBDIR, "libmmf.so")
 
 SOURCES = ["mmf_kernels.hip", "mmf_march.hip", "mmf_march2d.hip", "mmf_learn.hip", "mmf_api.cpp", "mmf_batch.cpp",
            "mmf_learn_api.cpp", "mmf_problem.cpp", "mmf_output.cpp", "mmf_noalloc.hip",
-           "mmf_mesh.hip", "mmf_mesh_api.cpp"]
+           "mmf_mesh.hip", "mmf_mesh_api.cpp", "mmf_march_o2.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so FP64 results round exactly as
 # the reference's expressions are written (bitwise parity with the oracle).

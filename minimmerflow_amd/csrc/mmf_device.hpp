@@ -118,11 +118,10 @@ struct Tup {
   double H;         // eto + p, eto = p/(gamma-1) + 0.5 rho |u|^2 (euler.cpp:103)
 };
 
-__device__ __forceinline__ void make_tup(const double *c, Tup &t) {
+// The tuple of state c from its primitives prim = cons2prim(c).
+__device__ __forceinline__ void make_tup_prim(const double *c, const double *prim, Tup &t) {
 #pragma unroll
   for (int k = 0; k < NF; ++k) t.c[k] = c[k];
-  double prim[NF];
-  cons2prim(c, prim);
   t.vel[0] = prim[FID_U];
   t.vel[1] = prim[FID_V];
   t.vel[2] = prim[FID_W];
@@ -131,6 +130,12 @@ __device__ __forceinline__ void make_tup(const double *c, Tup &t) {
   double vel2 = t.vel[0] * t.vel[0] + t.vel[1] * t.vel[1] + t.vel[2] * t.vel[2];  // euler.cpp:90
   double eto = t.p / (GAMMA - 1.) + 0.5 * c[FID_RHO] * vel2;         // euler.cpp:103
   t.H = eto + t.p;
+}
+
+__device__ __forceinline__ void make_tup(const double *c, Tup &t) {
+  double prim[NF];
+  cons2prim(c, prim);
+  make_tup_prim(c, prim, t);
 }
 
 // Mirror of the velocity component along axis a (euler.cpp:318-337 with

@@ -142,6 +142,9 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &shape, int sta
                               const double *R, const double *W, double *out_R, double *out_W, double *out_U,
                               Ctrl *ctrl, hipStream_t stream);
 bool march_shape_supported(const GridDev &g, const MarchShape &shape);
+// 3D order 2: the LDS-staged marching kernel (64 x 8 tiles), bodies included.
+hipError_t launch_stage_march3d_o2(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
+                                   const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s);
 // Tile table of the 3D marching kernel for cell flags 'flags' (cell order):
 // masked tiles first, the rest in the XCD-aware order.
 void build_march_tiles(const GridDev &g, const MarchShape &shape, const uint8_t *flags, int32_t *tiles);

@@ -61,6 +61,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
         const bool wall_p = ep1 && ((fp1 ^ fc) & CELL_FLUID);
 
         double Lm[NF], Rm[NF], Lp[NF], Rp[NF];  // face states: lower face (L|R), upper face (L|R)
+        Tup TLm2, TRm2, TLp2, TRp2;             // order 2: their tuples
         if (ORDER == 1) {
           // lower face: R = this cell
 #pragma unroll
@@ -112,8 +113,9 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
               sp[k] = minmod(dpp, dmp);
             }
           }
-          // reconstruction.cpp:67-97 extension: mean +/- slope/2, fallback to mean
-          auto recon = [&](const double *mean, const double *s, double sgn, double *v) {
+          // reconstruction.cpp:67-97 extension: mean +/- slope/2, fallback to
+          // mean; the tuple built from the primitives of the positivity check
+          auto recon = [&](const double *mean, const double *s, double sgn, double *v, Tup &T) {
 #pragma unroll
             for (int k = 0; k < NF; ++k) v[k] = (sgn > 0) ? mean[k] + 0.5 * s[k] : mean[k] - 0.5 * s[k];
             double pr[NF];
@@ -121,29 +123,35 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
             if (!(v[FID_RHO] > 0.0) || !(pr[FID_P] > 0.0)) {
 #pragma unroll
               for (int k = 0; k < NF; ++k) v[k] = mean[k];
+              cons2prim(v, pr);
             }
+            make_tup_prim(v, pr, T);
           };
-          recon(Xc, sc, -1.0, Rm);
-          recon(Xc, sc, +1.0, Lp);
+          recon(Xc, sc, -1.0, Rm, TRm2);
+          recon(Xc, sc, +1.0, Lp, TLp2);
           if (!em1) border_state(g, 2 * a, a, q, Rm, Lm);
           else if (wall_m) reflect_axis(Rm, a, Lm);
-          else recon(Xm1, sm, +1.0, Lm);
+          else recon(Xm1, sm, +1.0, Lm, TLm2);
           if (!ep1) border_state(g, 2 * a + 1, a, q, Lp, Rp);
           else if (wall_p) reflect_axis(Lp, a, Rp);
-          else recon(Xp1, sp, -1.0, Rp);
+          else recon(Xp1, sp, -1.0, Rp, TRp2);
+          if (!em1 || wall_m) make_tup(Lm, TLm2);
+          if (!ep1 || wall_p) make_tup(Rp, TRp2);
         }
 
         Tup TLm, TRm, TLp, TRp;
         if (ORDER == 1) {
           TRm = Tc;
           TLp = Tc;
+          // free-flow border and copy states reuse the computed tuple
+          make_tup(Lm, TLm);
+          make_tup(Rp, TRp);
         } else {
-          make_tup(Rm, TRm);
-          make_tup(Lp, TLp);
+          TRm = TRm2;
+          TLp = TLp2;
+          TLm = TLm2;
+          TRp = TRp2;
         }
-        // free-flow border and copy states reuse the computed tuple
-        make_tup(Lm, TLm);
-        make_tup(Rp, TRp);
 
         double Fm[NF], Fp[NF];
         double lm = face_flux(TLm, TRm, a, Fm);

@@ -727,7 +727,9 @@ void build_march_tiles(const GridDev &g, const MarchShape &sh, const uint8_t *fl
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
   if (sh.chunk < 1 || g.N >= MARCH_MAX_CELLS || g.bslot) return false;  // per-face borders: the gather kernel
-  // order 2: the 2D strip kernel, grids without bodies
+  // order 2: the 2D strip kernel (grids without bodies) and the 3D
+  // LDS-staged kernel (64 x 8 tiles)
+  if (g.order == 2 && g.dim == 3) return sh.tx == 64 && sh.ty == 8;
   if (g.order != 1 && !(g.order == 2 && g.dim == 2 && g.flags == nullptr)) return false;
   if (g.dim == 2) return sh.ty == 1;
   return (sh.tx == 64 && (sh.ty == 4 || sh.ty == 8)) || (sh.tx == 32 && (sh.ty == 8 || sh.ty == 16));
@@ -760,6 +762,7 @@ hipError_t launch_stage_march(const GridDev &g, const MarchShape &sh, int stage,
                               const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                               hipStream_t s) {
   const bool mask = g.flags != nullptr;
+  if (g.order == 2 && g.dim == 3) return launch_stage_march3d_o2(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   if (g.dim == 2) {
     if (g.order == 2) return launch_stage_march2d_o2(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
     return launch_stage_march2d(g, sh.chunk, stage, mode, U, R, W, oR, oW, oU, ctrl, s);

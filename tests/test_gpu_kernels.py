@@ -149,6 +149,43 @@ def test_march_order2_2d_equals_gather(mmf, orc, name, n, bc):
     assert ref[1] == mo
 
 
+O2_3D_CASES = [
+    ("3d o2 cube free", (48, 40, 36), [0] * 6, None),
+    ("3d o2 ragged mixed", (70, 13, 37), [0, 1, 2, 0, 1, 3], None),
+    ("3d o2 periodic xyz", (66, 10, 12), [4] * 6, None),
+    ("3d o2 periodic z thin", (5, 3, 9), [0, 1, 3, 0, 4, 4], None),
+    ("3d o2 dirichlet box", (40, 24, 20), [3, 0, 1, 1, 0, 2], {"boxes": [[-0.4, -0.3, -0.5, 0.1, 0.35, 0.2]]}),
+    ("3d o2 sphere", (36, 36, 36), [3, 0, 0, 0, 0, 0], {"spheres": [[0.1, 0.0, -0.1, 0.35]]}),
+    ("3d o2 periodic body", (130, 20, 24), [4, 4, 4, 4, 0, 3], {"boxes": [[0.9, -0.75, -0.9, 1.0, -0.68, -0.8]]}),
+]
+
+
+@pytest.mark.parametrize("name,n,bc,body", O2_3D_CASES, ids=[c[0] for c in O2_3D_CASES])
+def test_march_order2_3d_equals_gather(mmf, orc, name, n, bc, body):
+    """The 3D order-2 marching kernel (LDS-staged limited reconstruction,
+    each z face once) against the per-cell gather kernel: bit-identical RHS,
+    steps, time control and positivity counts for every chunk, bodies and
+    periodic axes included; and the oracle on the RHS."""
+    info = np.zeros((6, 5))
+    info[:] = [1.0, 0.3, -0.1, 0.05, 1.0]
+    s, U = build(mmf, 3, list(n), bc, info, body, order=2, seed=7)
+    ref = run_variant(mmf, s, U, mmf.KERNEL_GATHER)
+    for chunk in (32, 5, 1, 2, 1000):
+        got = run_variant(mmf, s, U, mmf.KERNEL_MARCH, 8, chunk)
+        assert s.kernel()[0] == mmf.KERNEL_MARCH
+        assert np.array_equal(got[0], ref[0]), (name, chunk, "rhs")
+        assert got[1] == ref[1], (name, chunk, "maxEig")
+        assert np.array_equal(got[2], ref[2]), (name, chunk, "state after 3 steps")
+        assert got[3] == ref[3], (name, chunk, "time control")
+        assert got[4] == ref[4], (name, chunk, "positivity counters")
+    fl = s.cell_flags() & 1
+    po = orc.problem(orc.RADSOD, 3)
+    o = orc.Solver(po, 2, list(n), [-1.0, -1.0, -1.0], 2.0 / max(n), side_bc=bc, side_info=info, fluid=fl)
+    Ro, mo = o.compute_rhs(U)
+    assert np.max(np.abs(ref[0] - Ro)) <= 1e-12 * np.max(np.abs(Ro))
+    assert ref[1] == mo
+
+
 @pytest.mark.parametrize("name,n,bc", O2_CASES, ids=[c[0] for c in O2_CASES])
 def test_tile_order2_2d_equals_gather(mmf, name, n, bc):
     """The 8x8 LDS tile kernel (ring cells recomputed, each face once, the
