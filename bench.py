@@ -723,7 +723,9 @@ def main():
             "kernel": "%s (fused flux + RK stage), mean of stages 1-3" % (
                 ("k_march_o2 (LDS-staged limited reconstruction) tile 64x8, %d planes/block" % kinfo[2]
                  if args.order == 2 else
-                 "k_march tile %dx%d, %d planes/block" % (args.cols or 64, kinfo[1], kinfo[2]))
+                 ("k_march tile %dx%d, %d planes/block" % (args.cols or 64, kinfo[1], kinfo[2]) if kinfo[2]
+                  else "k_march tile %dx%d, the planes split over one block per CU in shares of equal cost" % (args.cols or 64,
+                                                                                               kinfo[1])))
                 if kinfo[0] == mmf.KERNEL_MARCH
                 else "k_stage (per-cell gather)"),
             "algorithmic_bytes_per_launch": bytes_per_cell * N,

@@ -92,10 +92,19 @@ enum { MMF_KERNEL_AUTO = 0, MMF_KERNEL_GATHER = 1, MMF_KERNEL_MARCH = 2, MMF_KER
  * applies from 2^18 cells (2D order 1) or 2^20 cells (3D order 1, 2D order
  * 2); else GATHER.  mmf_set_kernel refuses MARCH or TILE on a grid they do not
  * apply to; bodies set afterwards make them fall back to GATHER.  tile_rows (3D: 4 or 8) and chunk (planes per block along the last axis)
- * shape the marching kernel; <= 0 keeps the default.  Every variant gives
- * bit-identical results. */
+ * shape the marching kernel; <= 0 keeps the default (3D order 1, 64 x 8
+ * tiles: where the chunks would leave a partial round of blocks, the planes
+ * of all tiles split over one block per CU in shares of equal cost, a plane
+ * that needs the masked code near a body counting 2.5 times).  Every variant
+ * gives bit-identical results. */
 int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk);
-/* kind actually used for the next launches, and the marching shape */
+/* 3D order 1, 64 x 8 tiles: the planes of all tiles split over 'blocks'
+ * blocks of the marching kernel in shares of equal cost (between the tile
+ * count and the tile planes; a block marches up to four segments, each of
+ * one tile and one code, masked or not); 0 restores the default. */
+int mmf_set_march_blocks(mmf_ctx *ctx, int32_t blocks);
+/* kind actually used for the next launches, and the marching shape (chunk 0:
+ * the planes split over the blocks in shares of equal cost) */
 int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_t *chunk);
 /* The 3D marching kernel's tile (tile_x x tile_y cells: 64 x 4, 64 x 8,
  * 32 x 8 or 32 x 16) and chunk; <= 0 keeps the default of that field.  Does

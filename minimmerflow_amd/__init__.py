@@ -151,6 +151,7 @@ def lib():
         "mmf_set_kernel": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "mmf_get_kernel": ([vp, i32p, i32p, i32p], C.c_int),
         "mmf_set_march_shape": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "mmf_set_march_blocks": ([vp, C.c_int32], C.c_int),
         "mmf_batch_create": ([C.POINTER(GridDesc), C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
         "mmf_batch_destroy": ([vp], C.c_int),
         "mmf_batch_last_error": ([vp], C.c_char_p),
@@ -425,6 +426,11 @@ class Solver:
     def set_march_shape(self, tile_x=0, tile_y=0, chunk=0):
         """3D marching tile (64 x 4, 64 x 8, 32 x 8, 32 x 16) and chunk; 0 keeps the default."""
         self._c(lib().mmf_set_march_shape(self.ctx, int(tile_x), int(tile_y), int(chunk)))
+
+    def set_march_blocks(self, blocks):
+        """3D order 1, 64 x 8 tiles: the planes of all tiles split over this many marching blocks in
+        shares of equal cost (0: the default)."""
+        self._c(lib().mmf_set_march_blocks(self.ctx, int(blocks)))
 
     def kernel(self):
         k, ty, ch = C.c_int32(), C.c_int32(), C.c_int32()

@@ -246,6 +246,7 @@ int mmf_create(const mmf_grid_desc *d, mmf_ctx **out) {
   if (hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
     ctx->num_cus = 0;
   ctx->shape = mmf::default_march_shape(g, ctx->num_cus);
+  mmf::balance_march_shape(g, ctx->num_cus, ctx->shape);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(MMF_ERR_HIP);
   const size_t state_bytes = sizeof(double) * 5 * (size_t)g.fs;
   if (hipMalloc(&ctx->dU, state_bytes) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
@@ -311,6 +312,7 @@ int mmf_set_kernel(mmf_ctx *ctx, int32_t kind, int32_t tile_rows, int32_t chunk)
   if (tile_rows > 0) sh.ty = tile_rows;
   if (chunk > 0) sh.chunk = chunk;
   if (tile_rows > 0 && sh.tx == 64 && tile_rows == 16 && ctx->g.dim == 3) sh.tx = 32;  // 64 x 16 does not fit; 32 x 16
+  if (chunk <= 0) mmf::balance_march_shape(ctx->g, ctx->num_cus, sh);
   if (kind == mmf::KERNEL_MARCH && !mmf::march_shape_supported(ctx->g, sh))
     return fail(ctx, MMF_ERR_INVALID, "marching kernel: order 1, or order 2 in 2D without bodies; fewer than 2^29 cells; tile rows 4 or 8 in 3D, 1 in 2D");
   ctx->kernel = kind;
@@ -325,8 +327,32 @@ int mmf_set_march_shape(mmf_ctx *ctx, int32_t tile_x, int32_t tile_y, int32_t ch
   sh.tx = tile_x > 0 ? tile_x : def.tx;
   sh.ty = tile_y > 0 ? tile_y : def.ty;
   sh.chunk = chunk > 0 ? chunk : def.chunk;
+  if (chunk > 0) sh.seg_blocks = 0;
+  else mmf::balance_march_shape(ctx->g, ctx->num_cus, sh);
   if (!mmf::march_shape_supported(ctx->g, sh))
     return fail(ctx, MMF_ERR_INVALID, "marching shape: 3D tiles 64 x 4, 64 x 8, 32 x 8 or 32 x 16; 2D 64 x 1");
+  ctx->lam_valid = false;
+  ctx->tiles_valid = false;
+  ctx->shape = sh;
+  return MMF_OK;
+}
+
+int mmf_set_march_blocks(mmf_ctx *ctx, int32_t blocks) {
+  if (!ctx || blocks < 0) return MMF_ERR_INVALID;
+  mmf::MarchShape sh = ctx->shape;
+  if (blocks == 0) {
+    sh = mmf::default_march_shape(ctx->g, ctx->num_cus);
+    sh.tx = ctx->shape.tx;
+    sh.ty = ctx->shape.ty;
+    mmf::balance_march_shape(ctx->g, ctx->num_cus, sh);
+  } else {
+    if (ctx->g.dim != 3 || ctx->g.order != 1 || sh.tx != 64 || sh.ty != 8)
+      return fail(ctx, MMF_ERR_INVALID, "march blocks: the 3D order-1 marching kernel, 64 x 8 tiles");
+    const int64_t ntiles = ((ctx->g.n[0] + sh.tx - 1) / sh.tx) * ((ctx->g.n[1] + sh.ty - 1) / sh.ty);
+    if (blocks < ntiles || blocks > ntiles * ctx->g.n[2])
+      return fail(ctx, MMF_ERR_INVALID, "march blocks: between the tile count and the tile planes");
+    sh.seg_blocks = blocks;
+  }
   ctx->lam_valid = false;
   ctx->tiles_valid = false;
   ctx->shape = sh;
@@ -337,7 +363,8 @@ int mmf_get_kernel(const mmf_ctx *ctx, int32_t *kind, int32_t *tile_rows, int32_
   if (!ctx) return MMF_ERR_INVALID;
   if (kind) *kind = kernel_used(ctx);
   if (tile_rows) *tile_rows = ctx->shape.ty;
-  if (chunk) *chunk = ctx->shape.chunk;
+  // 0: the planes of all tiles spread evenly over the blocks (3D order 1)
+  if (chunk) *chunk = (ctx->shape.seg_blocks > 0 && ctx->g.dim == 3 && ctx->g.order == 1) ? 0 : ctx->shape.chunk;
   return MMF_OK;
 }
 
@@ -795,16 +822,34 @@ constexpr int GRAPH_STEPS = 16;
 // The 3D marching kernel's tile table on grids with flags (the masked tiles
 // first; the others run the unmasked code).
 int ensure_tiles(mmf_ctx *ctx) {
-  const bool want = ctx->g.dim == 3 && ctx->g.flags != nullptr && !ctx->hflags.empty() &&
-                    kernel_used(ctx) == mmf::KERNEL_MARCH;
+  const bool march3 = ctx->g.dim == 3 && ctx->g.order == 1 && kernel_used(ctx) == mmf::KERNEL_MARCH;
+  const bool segs = march3 && ctx->shape.seg_blocks > 0;
+  const bool want = segs || (march3 && ctx->g.flags != nullptr && !ctx->hflags.empty());
   if (!want) {
     ctx->g.march_tiles = nullptr;
     return MMF_OK;
   }
   if (ctx->tiles_valid && ctx->g.march_tiles) return MMF_OK;
-  const int64_t nb = mmf::march_blocks(ctx->g, ctx->shape);
-  std::vector<int32_t> t((size_t)nb);
-  mmf::build_march_tiles(ctx->g, ctx->shape, ctx->hflags.data(), t.data());
+  // the segment table (3 * MARCH_SEGS entries per block) or the tile table
+  // (one); a share that needs more segments than a block has: the chunks
+  const uint8_t *fl = ctx->hflags.empty() ? nullptr : ctx->hflags.data();
+  std::vector<int32_t> t;
+  if (segs) {
+    t.resize((size_t)(ctx->shape.seg_blocks * 3 * mmf::MARCH_SEGS));
+    if (!mmf::build_march_segments(ctx->g, ctx->shape, fl, t.data())) {
+      ctx->shape.seg_blocks = 0;
+      if (ctx->g.flags == nullptr || ctx->hflags.empty()) {
+        ctx->g.march_tiles = nullptr;
+        return MMF_OK;
+      }
+      t.clear();
+    }
+  }
+  if (t.empty()) {
+    t.resize((size_t)mmf::march_blocks(ctx->g, ctx->shape));
+    mmf::build_march_tiles(ctx->g, ctx->shape, fl, t.data());
+  }
+  const int64_t nb = (int64_t)t.size();
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->dtiles_n != nb) {
     if (ctx->dtiles) (void)hipFree(ctx->dtiles);

@@ -65,7 +65,14 @@ struct MarchShape {
   int ty;       // tile rows (3D): 4, 8 or 16; 1 in 2D
   int chunk;    // planes per block along the march axis
   int tx = 64;  // tile columns (3D): 64 (one wave per row) or 32
+  // 3D order 1: > 0 spreads the planes of all tiles over this many blocks
+  // in shares of equal cost (up to MARCH_SEGS segments each, from a segment
+  // table) instead of one (tile, chunk) per block; 0 keeps the chunks
+  int seg_blocks = 0;
 };
+
+// Segments per block in the 3D marching kernel's segment table.
+constexpr int MARCH_SEGS = 4;
 
 // Stage maxima and block counters are spread over EIG_SPLIT cache lines each
 // (block b uses line b % EIG_SPLIT): device-scope atomics on one address
@@ -148,6 +155,16 @@ hipError_t launch_stage_march3d_o2(const GridDev &g, int chunk, int stage, int m
 // Tile table of the 3D marching kernel for cell flags 'flags' (cell order):
 // masked tiles first, the rest in the XCD-aware order.
 void build_march_tiles(const GridDev &g, const MarchShape &shape, const uint8_t *flags, int32_t *tiles);
+// The segment table of shape.seg_blocks blocks (3 * MARCH_SEGS entries
+// each): the tile planes split into shares of equal cost (a plane that needs
+// the masked code costs more), XCD-aware, each share cut into runs of one
+// tile and one code; per block its runs as (tile | MARCH_TILE_MASKED, first
+// plane, end plane), -1 after the last.  flags (cell order) or nullptr.
+// False when a share would need more than MARCH_SEGS runs of different tiles.
+bool build_march_segments(const GridDev &g, const MarchShape &shape, const uint8_t *flags, int32_t *table);
+// Sets shape.seg_blocks where the chunks leave a partial round of blocks on
+// num_cus CUs and every share stays within a few tiles.
+void balance_march_shape(const GridDev &g, int num_cus, MarchShape &shape);
 int64_t march_blocks(const GridDev &g, const MarchShape &shape);
 // The marching kernel takes dt from Ctrl::lam_now (early dt) on this grid.
 bool march_early(const GridDev &g);

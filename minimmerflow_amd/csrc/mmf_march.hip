@@ -213,15 +213,12 @@ __device__ __forceinline__ double cell_max_eig(const double *c) {
 // and bct are the kernel's LDS arrays.
 template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool EARLY>
 __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in, const double *U, double *out,
-                                            Ctrl *ctrl, int mode, int chunk, int ntx, int nty, int tile,
-                                            double *lds, uint8_t *lfl, double *bct) {
-  constexpr int NT = TX * TY;
+                                            Ctrl *ctrl, int mode, int bx, int by, int m0, int m1, double *lds,
+                                            uint8_t *lfl, double *bct, double &lam_max, double &lam_nx,
+                                            NegCount &neg) {
   constexpr int M = DIM - 1;  // march axis
   constexpr int RW = TX + 2;
   constexpr int PL = (TY + 2) * RW;
-
-  const int slot_eig = (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1;
-  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
   // Early dt (no body, no reflected border): the step's dt is known before
   // stage 1 (Ctrl::lam_now), stage 1 writes W1 = U + dt R / V itself, stage 2
   // reads W1 like stage 3 reads W, and stage 3 gathers the next step's
@@ -242,10 +239,6 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   OutFields outf;
   outf.set(out, g.fs);
 
-  const int bx = tile % ntx;
-  const int by = (tile / ntx) % nty;
-  const int bz = tile / (ntx * nty);
-
   const int lane = threadIdx.x, row = threadIdx.y;
   const int tid = row * TX + lane;
   const int nx = (int)g.n[0];
@@ -253,8 +246,6 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   const int nm = (int)g.n[M];
   const int x0 = bx * TX;
   const int y0 = by * TY;
-  const int m0 = bz * chunk;
-  const int m1 = (m0 + chunk < nm) ? m0 + chunk : nm;
   const int w = (nx - x0 < TX) ? nx - x0 : TX;
   const int hgt = (DIM == 3) ? ((ny - y0 < TY) ? ny - y0 : TY) : 1;
   const bool valid = lane < w && row < hgt;
@@ -315,9 +306,6 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   gp.pow2V = P2;
   const double A = g.A, halfA = g.halfA;
 
-  double lam_max = 0.0;
-  double lam_nx = 0.0;  // early dt, stage 3: the next step's maximum over this thread's cells
-  NegCount neg;  // non-positive states met, summed into the control block at the end
   Tup Tc;  // tuple of this column's cell in plane m
   uint8_t fc = CELL_FLUID | CELL_SOLVED;
   bool bad_c = false;   // Tc has a negative p or rho
@@ -562,10 +550,70 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
     __syncthreads();
     buf ^= 1;
   }
+}
+
+// TX x TY tiles: 64 x 4 / 64 x 8 (one wave per row) and 32 x 8 (four waves,
+// 60 KB of LDS: two independent blocks per CU, each SIMD holding a wave of
+// each, so one block's plane barrier and division chains overlap the other's
+// work) or 32 x 16 (eight waves, a halo ring of 19% of the tile's cells
+// instead of 28% for 64 x 8).
+template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool SEGS>
+__global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
+                                                     Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
+  constexpr int PL = (TY + 2) * (TX + 2);
+  static_assert(2 * NTUP * PL * 8 <= 120 * 1024, "two tuple planes must fit in LDS");
+  __shared__ double lds[2 * NTUP * PL];
+  __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
+  __shared__ double bct[6 * NTUP];
+  constexpr bool EARLY = !MASK && !REFL;
+  constexpr int NT = TX * TY;
+  if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int nm = (int)g.n[DIM - 1];
+  double lam_max = 0.0;
+  double lam_nx = 0.0;  // early dt, stage 3: the next step's maximum over this thread's cells
+  NegCount neg;         // non-positive states met, summed into the control block at the end
+  bool any = false;     // this block marched a segment (and built bct)
+  if constexpr (SEGS) {
+    // the segment table: up to MARCH_SEGS (tile | masked, first plane, end
+    // plane) per block, unused ones -1 at the end.  One call site per
+    // variant: the march is inlined once masked and once not.
+#pragma unroll 1
+    for (int k = 0; k < MARCH_SEGS; ++k) {
+      const int32_t *t = g.march_tiles + 3 * (MARCH_SEGS * (int64_t)blockIdx.x + k);
+      if (t[0] < 0) break;
+      any = true;
+      const int tile = t[0] & ~MARCH_TILE_MASKED;
+      const int bx = tile % ntx, by = tile / ntx;
+      if (MASK && (t[0] & MARCH_TILE_MASKED))
+        march_block<DIM, TX, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, bx, by, t[1], t[2], lds,
+                                                               lfl, bct, lam_max, lam_nx, neg);
+      else
+        march_block<DIM, TX, TY, STAGE, false, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, bx, by, t[1], t[2], lds,
+                                                                lfl, bct, lam_max, lam_nx, neg);
+    }
+  } else {
+    // one (tile, chunk) per block, from the tile table (masked tiles first; a
+    // tile whose cells, halo ring and neighbour planes are all fluid and
+    // solved runs the unmasked code) or the XCD-aware numbering
+    any = true;
+    const int t = (MASK && g.march_tiles != nullptr) ? g.march_tiles[blockIdx.x]
+                                                     : march_tile_of_block((int)blockIdx.x, (int)gridDim.x);
+    const int tt = t & ~MARCH_TILE_MASKED;
+    const int a = tt / (ntx * nty) * chunk;
+    const int b = (a + chunk < nm) ? a + chunk : nm;
+    const int bx = tt % ntx, by = (tt / ntx) % nty;
+    if (MASK && (g.march_tiles == nullptr || (t & MARCH_TILE_MASKED)))
+      march_block<DIM, TX, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, bx, by, a, b, lds, lfl, bct,
+                                                             lam_max, lam_nx, neg);
+    else
+      march_block<DIM, TX, TY, STAGE, false, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, bx, by, a, b, lds, lfl, bct,
+                                                              lam_max, lam_nx, neg);
+  }
   neg.flush(ctrl, tid);
-  block_max_to<NT>(ctrl, slot_eig, lam_max, tid);
+  block_max_to<NT>(ctrl, (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1, lam_max, tid);
   if (EARLY && STAGE == 3 && mode == MODE_STEP) {
-    if (tid == 0) {  // a Dirichlet border's virtual state enters the next maximum too
+    if (tid == 0 && any) {  // a Dirichlet border's virtual state enters the next maximum too
 #pragma unroll
       for (int side = 0; side < 2 * DIM; ++side)
         if (g.bc[side] == BC_DIRICHLET && !g.periodic[side / 2]) {
@@ -577,37 +625,6 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   }
   // stage 3 of a step: the last block advances the time (no separate launch)
   if (STAGE == 3 && mode == MODE_STEP) close_step<EARLY>(ctrl, tid);
-}
-
-// TX x TY tiles: 64 x 4 / 64 x 8 (one wave per row) and 32 x 8 (four waves,
-// 60 KB of LDS: two independent blocks per CU, each SIMD holding a wave of
-// each, so one block's plane barrier and division chains overlap the other's
-// work) or 32 x 16 (eight waves, a halo ring of 19% of the tile's cells
-// instead of 28% for 64 x 8).
-template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL>
-__global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
-                                                     Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
-  constexpr int PL = (TY + 2) * (TX + 2);
-  static_assert(2 * NTUP * PL * 8 <= 120 * 1024, "two tuple planes must fit in LDS");
-  __shared__ double lds[2 * NTUP * PL];
-  __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
-  __shared__ double bct[6 * NTUP];
-  constexpr bool EARLY = !MASK && !REFL;
-  if (MASK && g.march_tiles != nullptr) {
-    // the tile table: masked tiles first; a tile whose cells, halo ring and
-    // neighbour planes are all fluid and solved runs the unmasked code
-    const int t = g.march_tiles[blockIdx.x];
-    if (t & MARCH_TILE_MASKED)
-      march_block<DIM, TX, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
-                                                             t & ~MARCH_TILE_MASKED, lds, lfl, bct);
-    else
-      march_block<DIM, TX, TY, STAGE, false, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty, t, lds,
-                                                              lfl, bct);
-  } else {
-    march_block<DIM, TX, TY, STAGE, MASK, P2, REFL, EARLY>(g, in, U, out, ctrl, mode, chunk, ntx, nty,
-                                                       march_tile_of_block((int)blockIdx.x, (int)gridDim.x), lds,
-                                                       lfl, bct);
-  }
 }
 
 template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL>
@@ -626,11 +643,19 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   const int M = DIM - 1;
   const int ntx = (int)((g.n[0] + TX - 1) / TX);
   const int nty = (DIM == 3) ? (int)((g.n[1] + TY - 1) / TY) : 1;
-  const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
-  dim3 grid((unsigned)(ntx * nty * nch));
   dim3 block(TX, TY);
-  hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL>), grid, block, 0, s, g, in, U, out, ctrl, mode,
-                     sh.chunk, ntx, nty);
+  // the segment table when one was built for this shape (64 x 8 tiles), else
+  // one (tile, chunk) per block
+  if constexpr (DIM == 3 && TX == 64 && TY == 8) {
+    if (sh.seg_blocks > 0 && g.march_tiles != nullptr) {
+      hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL, true>), dim3((unsigned)sh.seg_blocks), block,
+                         0, s, g, in, U, out, ctrl, mode, 0, ntx, nty);
+      return hipGetLastError();
+    }
+  }
+  const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
+  hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL, false>), dim3((unsigned)(ntx * nty * nch)), block,
+                     0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx, nty);
   return hipGetLastError();
 }
 
@@ -667,6 +692,7 @@ static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int sta
 }
 
 int64_t march_blocks(const GridDev &g, const MarchShape &sh) {
+  if (sh.seg_blocks > 0) return sh.seg_blocks;
   const int TX = sh.tx;
   const int64_t ntx = (g.n[0] + TX - 1) / TX, nty = (g.n[1] + sh.ty - 1) / sh.ty;
   return ntx * nty * ((g.n[2] + sh.chunk - 1) / sh.chunk);
@@ -723,6 +749,119 @@ void build_march_tiles(const GridDev &g, const MarchShape &sh, const uint8_t *fl
         }
       }
   }
+}
+
+// Per plane m of tile (bx, by): the tile's cells and halo ring in planes m -
+// 1, m and m + 1 (wrapped across periodic sides) are not all fluid and solved,
+// i.e. marching plane m needs the masked code.
+static void plane_masks(const GridDev &g, int TX, int TY, int64_t bx, int64_t by, const uint8_t *flags,
+                        std::vector<uint8_t> &pm) {
+  const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  pm.assign((size_t)nz, 0);
+  if (!flags) return;
+  const uint8_t all = CELL_FLUID | CELL_SOLVED;
+  auto wrap = [](int64_t i, int64_t n, bool per, int64_t &out) {
+    if (i >= 0 && i < n) { out = i; return true; }
+    if (!per) return false;
+    out = (i + n) % n;
+    return true;
+  };
+  const int64_t x0 = bx * TX, y0 = by * TY;
+  const int64_t x1 = std::min(x0 + TX, nx), y1 = std::min(y0 + TY, ny);
+  std::vector<uint8_t> ring((size_t)nz, 0);  // plane z's tile and ring
+  for (int64_t z = 0; z < nz; ++z) {
+    bool m = false;
+    for (int64_t y = y0 - 1; y <= y1 && !m; ++y) {
+      int64_t yy;
+      if (!wrap(y, ny, g.periodic[1], yy)) continue;
+      const uint8_t *row = flags + (yy + ny * z) * nx;
+      for (int64_t x = x0 - 1; x <= x1; ++x) {
+        int64_t xx;
+        if (!wrap(x, nx, g.periodic[0], xx)) continue;
+        if (row[xx] != all) { m = true; break; }
+      }
+    }
+    ring[(size_t)z] = m;
+  }
+  for (int64_t m = 0; m < nz; ++m)
+    for (int64_t z = m - 1; z <= m + 1; ++z) {
+      int64_t zz;
+      if (wrap(z, nz, g.periodic[2], zz) && ring[(size_t)zz]) pm[(size_t)m] = 1;
+    }
+}
+
+// Cost of a plane marched by the unmasked and by the masked code (flags,
+// body faces), in quarters of the former: on a 192^3 grid with a sphere the
+// stage time is lowest with masked planes at 2.5 to 3.5 times an unmasked one.
+constexpr int64_t PLANE_COST = 4, PLANE_COST_MASKED = 10;
+
+bool build_march_segments(const GridDev &g, const MarchShape &sh, const uint8_t *flags, int32_t *table) {
+  const int64_t nm = g.n[2];
+  const int64_t ntx = (g.n[0] + sh.tx - 1) / sh.tx, nty = (g.n[1] + sh.ty - 1) / sh.ty;
+  const int64_t W = ntx * nty * nm;  // tile planes, tile by tile
+  const int B = sh.seg_blocks;
+  // masked planes and the cost prefix over the tile planes
+  std::vector<uint8_t> masked((size_t)W), pm;
+  std::vector<int64_t> pre((size_t)W + 1, 0);
+  for (int64_t t = 0; t < ntx * nty; ++t) {
+    plane_masks(g, sh.tx, sh.ty, t % ntx, t / ntx, flags, pm);
+    for (int64_t m = 0; m < nm; ++m) masked[(size_t)(t * nm + m)] = pm[(size_t)m];
+  }
+  for (int64_t i = 0; i < W; ++i) pre[(size_t)i + 1] = pre[(size_t)i] + (masked[(size_t)i] ? PLANE_COST_MASKED : PLANE_COST);
+  const int64_t total = pre[(size_t)W];
+  // first tile plane of block share L: the first whose cost prefix reaches L / B of the total
+  auto first = [&](int64_t L) {
+    const int64_t goal = (L * total + B - 1) / B;  // ceil: shares of equal cost, boundaries exact at L = B
+    return (int64_t)(std::lower_bound(pre.begin(), pre.end(), goal) - pre.begin());
+  };
+  struct Seg { int64_t tile, m0, m1; bool masked; };
+  std::vector<Seg> segs;
+  for (int b = 0; b < B; ++b) {
+    // XCD-aware: the blocks of one XCD (b % 8) take a contiguous range of tiles
+    const int64_t L = march_tile_of_block(b, B);
+    const int64_t start = std::min(first(L), W), end = std::min(first(L + 1), W);
+    // runs of one tile and one code (masked or not)
+    segs.clear();
+    for (int64_t i = start; i < end; ++i) {
+      const int64_t tile = i / nm, m = i % nm;
+      const bool mk = masked[(size_t)i];
+      if (!segs.empty() && segs.back().tile == tile && segs.back().masked == mk) segs.back().m1 = m + 1;
+      else segs.push_back({tile, m, m + 1, mk});
+    }
+    // too many: merge the shortest neighbour pair of one tile (masked if either is)
+    while ((int)segs.size() > MARCH_SEGS) {
+      int best = -1;
+      int64_t len = 0;
+      for (size_t k = 0; k + 1 < segs.size(); ++k)
+        if (segs[k].tile == segs[k + 1].tile && (best < 0 || segs[k + 1].m1 - segs[k].m0 < len)) {
+          best = (int)k;
+          len = segs[k + 1].m1 - segs[k].m0;
+        }
+      if (best < 0) return false;  // more tiles than segments
+      segs[(size_t)best].m1 = segs[(size_t)best + 1].m1;
+      segs[(size_t)best].masked = segs[(size_t)best].masked || segs[(size_t)best + 1].masked;
+      segs.erase(segs.begin() + best + 1);
+    }
+    int32_t *t = table + 3 * MARCH_SEGS * (int64_t)b;
+    for (int k = 0; k < 3 * MARCH_SEGS; ++k) t[k] = -1;
+    for (size_t k = 0; k < segs.size(); ++k) {
+      t[3 * k] = (int32_t)segs[k].tile | (segs[k].masked ? MARCH_TILE_MASKED : 0);
+      t[3 * k + 1] = (int32_t)segs[k].m0;
+      t[3 * k + 2] = (int32_t)segs[k].m1;
+    }
+  }
+  return true;
+}
+
+void balance_march_shape(const GridDev &g, int num_cus, MarchShape &sh) {
+  sh.seg_blocks = 0;
+  if (g.dim != 3 || g.order != 1 || num_cus <= 0 || sh.tx != 64 || sh.ty != 8) return;
+  const int64_t nb = march_blocks(g, sh);
+  if (nb % num_cus == 0) return;  // the chunks make whole rounds
+  const int64_t ntiles = ((g.n[0] + sh.tx - 1) / sh.tx) * ((g.n[1] + sh.ty - 1) / sh.ty);
+  // a block's share within about two tiles, and at least 8 planes
+  if (ntiles > num_cus || ntiles * g.n[2] < 8 * (int64_t)num_cus) return;
+  sh.seg_blocks = num_cus;
 }
 
 bool march_shape_supported(const GridDev &g, const MarchShape &sh) {

@@ -57,6 +57,8 @@ CASES = [
     # a body against the periodic x and y sides: the tiles across the wrap read it
     ("3d periodic body", 3, (130, 20, 24), [4, 4, 4, 4, 0, 3],
      {"boxes": [[0.9, -0.75, -0.9, 1.0, -0.68, -0.8]]}),
+    # a body on the periodic z sides: the planes across the wrap need the masked code
+    ("3d periodic z body", 3, (70, 18, 40), [0, 3, 1, 0, 4, 4], {"boxes": [[-0.2, -0.9, -1.0, 0.3, -0.7, -0.95]]}),
     ("2d free", 2, (64, 64), [0] * 4, None),
     ("2d ragged periodic", 2, (100, 37), [4, 4, 4, 4], None),
     ("2d box reflecting", 2, (77, 50), [1, 1, 1, 1], {"boxes": [[-0.2, -0.2, -1, 0.3, 0.1, 1]]}),
@@ -78,6 +80,27 @@ def test_march_equals_gather(mmf, orc, name, dim, n, bc, body):
         assert np.array_equal(got[2], ref[2]), (name, rows, chunk, "state after 3 steps")
         assert got[3] == ref[3], (name, rows, chunk, "time control")
         assert got[4] == ref[4], (name, rows, chunk, "positivity counters")
+    if dim == 3:
+        # the planes split over a number of blocks in shares of equal cost
+        # (segments that start and end at arbitrary planes of a tile, cut
+        # where the code changes between masked and unmasked)
+        ntiles = -(-n[0] // 64) * -(-n[1] // 8)
+        for blocks in sorted({ntiles, ntiles + 3, 2 * ntiles + 1, 7 * ntiles - 2, ntiles * n[2]}):
+            s.set_kernel(mmf.KERNEL_MARCH, 8)
+            s.set_march_blocks(blocks)
+            assert s.kernel()[2] == 0
+            s.set_state(U)
+            R, me = s.compute_rhs()
+            s.set_time(0.0, 10.0, 0.5)
+            s.advance(3)
+            t, dt, step, eig = s.time()
+            got = (R, me, s.get_state(), (t, dt, step, tuple(eig)), s.positivity_counters())
+            assert np.array_equal(got[0], ref[0]), (name, blocks, "rhs")
+            assert got[1] == ref[1], (name, blocks, "maxEig")
+            assert np.array_equal(got[2], ref[2]), (name, blocks, "state after 3 steps")
+            assert got[3] == ref[3], (name, blocks, "time control")
+            assert got[4] == ref[4], (name, blocks, "positivity counters")
+        s.set_march_blocks(0)
     # and the oracle on the RHS
     fl = s.cell_flags() & 1
     po = orc.problem(orc.RADSOD, dim)
