@@ -370,7 +370,11 @@ int mmf_batch_get_state(mmf_batch *b, int32_t ci, double *cons);
 int mmf_batch_set_initial_states(mmf_batch *b, const mmf_problem *problems);
 /* per-case start time, end time and CFL (arrays of ncases) */
 int mmf_batch_set_time(mmf_batch *b, const double *t0, const double *tmax, const double *cfl);
-/* rows per work item and persistent grid size (<= 0 keeps the current value) */
+/* rows per work item and persistent grid size of stages 2 and 3 (<= 0 keeps
+ * the current value; stage 1, resident at three waves per SIMD where they
+ * are at two, runs proportionally more blocks).  The items are cut into one
+ * queue per XCD, runs of neighbouring strips and row bands of equal cost; a
+ * block takes items from its XCD's queue, then from the others. */
 int mmf_batch_set_shape(mmf_batch *b, int32_t chunk, int32_t blocks);
 int mmf_batch_get_shape(const mmf_batch *b, int32_t *chunk, int32_t *blocks);
 /* rows of the small items that end each stage: the last 'chunk' rows of

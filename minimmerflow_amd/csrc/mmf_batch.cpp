@@ -39,10 +39,12 @@ struct mmf_batch {
   BatchCase *dcases = nullptr;
   int *ditems = nullptr;
   int nitems = 0;
+  mmf::BatchQueues queues{};
   unsigned int *dcounters = nullptr;
   int chunk = 16;
   int tail = 0;  // rows of the small items at the end of the list (0: none)
   int blocks = 0;
+  int occ[2] = {0, 0};  // resident blocks per CU of stages 1 and 2
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double *dpartials = nullptr;
@@ -79,8 +81,17 @@ void drop_graph(mmf_batch *b) {
 // every strip become items of 'tail' rows, queued after all full items, so
 // each stage drains in small pieces instead of ending on the slowest full
 // item.
+// The work items (case, strip, first row, end row) in BATCH_QUEUES queues,
+// one per XCD.  In case order, row band by row band, strip by strip, the
+// items are cut into contiguous runs of equal cost (cells plus the two halo
+// rows an item reads), one run per queue, so the strips of a row band and
+// the bands above and below it go to one XCD, whose L2 then serves the lines
+// they share.  Each queue holds its full items (chunk rows) first and then,
+// with a tail, each strip's last chunk rows as items of tail rows.
 int build_items(mmf_batch *b) {
-  std::vector<int> items, small;
+  struct Item { int c, t, m0, m1; bool small; };
+  std::vector<Item> all;
+  int64_t total = 0;
   for (int i = 0; i < b->ncases; ++i) {
     BatchCase &c = b->hcases[(size_t)i];
     c.ntx = mmf::march2d_tiles(c.g.n[0]);
@@ -88,25 +99,46 @@ int build_items(mmf_batch *b) {
     const int ny = (int)c.g.n[1];
     const int split = (b->tail > 0) ? std::max(0, ny - b->chunk) : ny;  // rows of full items
     for (int m0 = 0; m0 < split; m0 += b->chunk)
-      for (int t = 0; t < c.ntx; ++t) items.insert(items.end(), {i, t, m0, std::min(m0 + b->chunk, split)});
+      for (int t = 0; t < c.ntx; ++t) all.push_back({i, t, m0, std::min(m0 + b->chunk, split), false});
     for (int m0 = split; m0 < ny; m0 += b->tail)
-      for (int t = 0; t < c.ntx; ++t) small.insert(small.end(), {i, t, m0, std::min(m0 + b->tail, ny)});
+      for (int t = 0; t < c.ntx; ++t) all.push_back({i, t, m0, std::min(m0 + b->tail, ny), true});
   }
-  items.insert(items.end(), small.begin(), small.end());
+  for (const Item &it : all) total += it.m1 - it.m0 + 2;
+  std::vector<int> q[mmf::BATCH_QUEUES][2];  // per queue: full, then small items
+  int64_t acc = 0;
+  for (const Item &it : all) {
+    const int k = (int)std::min<int64_t>(mmf::BATCH_QUEUES - 1, acc * mmf::BATCH_QUEUES / std::max<int64_t>(total, 1));
+    q[k][it.small ? 1 : 0].insert(q[k][it.small ? 1 : 0].end(), {it.c, it.t, it.m0, it.m1});
+    acc += it.m1 - it.m0 + 2;
+  }
+  std::vector<int> items;
+  for (int k = 0; k < mmf::BATCH_QUEUES; ++k) {
+    b->queues.beg[k] = (int)(items.size() / 4);
+    for (int h = 0; h < 2; ++h) items.insert(items.end(), q[k][h].begin(), q[k][h].end());
+  }
+  b->queues.beg[mmf::BATCH_QUEUES] = (int)(items.size() / 4);
   b->nitems = (int)(items.size() / 4);
   if (b->ditems) (void)hipFree(b->ditems);
   b->ditems = nullptr;
-  BCHK(b, hipMalloc(&b->ditems, sizeof(int) * items.size()));
-  BCHK(b, hipMemcpy(b->ditems, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
+  BCHK(b, hipMalloc(&b->ditems, sizeof(int) * std::max<size_t>(items.size(), 4)));
+  if (!items.empty())
+    BCHK(b, hipMemcpy(b->ditems, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
   BCHK(b, hipMemcpy(b->dcases, b->hcases.data(), sizeof(BatchCase) * (size_t)b->ncases, hipMemcpyHostToDevice));
   drop_graph(b);
   return MMF_OK;
 }
 
+// Persistent blocks of stage s: 'blocks' for stages 2 and 3; stage 1, which
+// holds three waves per SIMD where they hold two, proportionally more.
+int stage_blocks(const mmf_batch *b, int s) {
+  if (s != 1 || b->occ[0] <= 0 || b->occ[1] <= 0) return b->blocks;
+  return (int)((int64_t)b->blocks * b->occ[0] / b->occ[1]);
+}
+
 int enqueue_step(mmf_batch *b) {
   for (int s = 1; s <= 3; ++s) {
-    hipError_t e = mmf::launch_batch_stage(s, b->dcases, b->ditems, b->nitems, b->dcounters, b->dU, b->dR, b->dW,
-                                           b->blocks, b->stream);
+    hipError_t e = mmf::launch_batch_stage(s, b->dcases, b->ditems, b->queues, b->dcounters, b->dU, b->dR, b->dW,
+                                           stage_blocks(b, s), b->stream);
     if (e != hipSuccess) return bfail(b, MMF_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
   }
   hipError_t e = mmf::launch_batch_advance(b->dcases, b->ncases, b->dcounters, b->stream);
@@ -198,6 +230,8 @@ int mmf_batch_create(const mmf_grid_desc *cases, int32_t ncases, mmf_batch **out
   if (hipGetDevice(&dev) != hipSuccess) return cleanup(MMF_ERR_NO_DEVICE);
   if (hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->num_cus = 256;
   b->blocks = 8 * b->num_cus;
+  b->occ[0] = mmf::batch_stage_occupancy(1);
+  b->occ[1] = mmf::batch_stage_occupancy(2);
   int64_t off = 0;
   for (int i = 0; i < ncases; ++i) {
     const mmf_grid_desc &d = cases[i];
@@ -241,10 +275,11 @@ int mmf_batch_create(const mmf_grid_desc *cases, int32_t ncases, mmf_batch **out
   if (hipHostMalloc(&b->hctrl, sizeof(Ctrl) * (size_t)ncases, hipHostMallocDefault) != hipSuccess)
     return cleanup(MMF_ERR_OUT_OF_MEMORY);
   if (hipMalloc(&b->dcases, sizeof(BatchCase) * (size_t)ncases) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
-  if (hipMalloc(&b->dcounters, sizeof(unsigned int) * 4) != hipSuccess) return cleanup(MMF_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&b->dcounters, sizeof(unsigned int) * mmf::BATCH_COUNTERS) != hipSuccess)
+    return cleanup(MMF_ERR_OUT_OF_MEMORY);
   if (hipMemset(b->dU, 0, bytes) != hipSuccess || hipMemset(b->dR, 0, bytes) != hipSuccess ||
       hipMemset(b->dW, 0, bytes) != hipSuccess || hipMemset(b->dctrl, 0, sizeof(Ctrl) * (size_t)ncases) != hipSuccess ||
-      hipMemset(b->dcounters, 0, sizeof(unsigned int) * 4) != hipSuccess)
+      hipMemset(b->dcounters, 0, sizeof(unsigned int) * mmf::BATCH_COUNTERS) != hipSuccess)
     return cleanup(MMF_ERR_HIP);
   std::memset(b->hctrl, 0, sizeof(Ctrl) * (size_t)ncases);
   for (int i = 0; i < ncases; ++i) b->hcases[(size_t)i].ctrl = b->dctrl + i;

@@ -23,8 +23,20 @@ hipError_t launch_stage_march2d(const GridDev &g, int chunk, int stage, int mode
 hipError_t launch_stage_march2d_o2(const GridDev &g, int chunk, int stage, int mode, const double *U, const double *R,
                                    const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s);
 int march2d_tiles(int64_t nx);
-hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,
-                              double *U, double *R, double *W, int nblocks, hipStream_t s);
+// Item queues of the persistent batch kernel: one per XCD (a block takes
+// items from the queue of blockIdx.x % BATCH_QUEUES, the XCD it runs on, and
+// then from the others); queue q holds items [beg[q], beg[q + 1]).  Each
+// queue's counter sits on its own 128-byte line, per stage.
+constexpr int BATCH_QUEUES = 8;
+constexpr int BATCH_COUNTER_LINE = 32;  // unsigned ints per line
+constexpr int BATCH_COUNTERS = 3 * BATCH_QUEUES * BATCH_COUNTER_LINE;
+struct BatchQueues {
+  int beg[BATCH_QUEUES + 1];
+};
+hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, const BatchQueues &queues,
+                              unsigned int *counters, double *U, double *R, double *W, int nblocks, hipStream_t s);
+// Resident blocks per CU of the stage's batch kernel (0 if unknown).
+int batch_stage_occupancy(int stage);
 hipError_t launch_batch_advance(const BatchCase *cases, int ncases, unsigned int *counters, hipStream_t s);
 
 }  // namespace mmf

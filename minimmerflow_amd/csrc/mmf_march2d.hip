@@ -969,17 +969,30 @@ hipError_t launch_stage_march2d_o2(const GridDev &g, int chunk, int stage, int m
 // scratch): 2% faster per step on the 512-case set; stages 2 and 3 need more
 // registers and measured slower at three.
 template <int STAGE>
-__global__ __launch_bounds__(W2, (STAGE == 1) ? 3 : 2) void k_batch2d(const BatchCase *cases, const int *items, int nitems,
-                                                unsigned int *counter, double *U, double *R, double *W) {
+__global__ __launch_bounds__(W2, (STAGE == 1) ? 3 : 2) void k_batch2d(const BatchCase *cases, const int *items,
+                                                                      BatchQueues qs, unsigned int *counters,
+                                                                      double *U, double *R, double *W) {
   __shared__ double lds[2 * NTUP * W2];
   __shared__ uint8_t lfl[1];
   __shared__ int item_s;
+  // this block's XCD's queue first (its neighbouring strips share lines in
+  // that XCD's L2), then the others in turn once it is drained
+  const int home = (int)(blockIdx.x % BATCH_QUEUES);
+  int k = 0;
   for (;;) {
-    if (threadIdx.x == 0) item_s = (int)atomicAdd(counter, 1u);
+    const int q = (home + k) % BATCH_QUEUES;
+    if (threadIdx.x == 0) {
+      const int n = qs.beg[q + 1] - qs.beg[q];
+      const int i = (int)atomicAdd(counters + q * BATCH_COUNTER_LINE, 1u);
+      item_s = (i < n) ? qs.beg[q] + i : -1;
+    }
     __syncthreads();
     const int it = item_s;
     __syncthreads();
-    if (it >= nitems) break;  // every wave of the block reaches this exit
+    if (it < 0) {  // queue q is drained: the next one, until all are
+      if (++k == BATCH_QUEUES) break;  // every wave of the block reaches this exit
+      continue;
+    }
     // item: case, strip, rows [m0, m1)
     const int ci = items[4 * it], tile = items[4 * it + 1], m0 = items[4 * it + 2], m1 = items[4 * it + 3];
     const BatchCase &bc = cases[ci];
@@ -1010,7 +1023,7 @@ __global__ __launch_bounds__(W2, (STAGE == 1) ? 3 : 2) void k_batch2d(const Batc
 // t += dt per case (main.cpp:505-506), resets the stage maxima and the item counters.
 __global__ void k_batch_advance(const BatchCase *cases, int ncases, unsigned int *counters) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3 && blockIdx.x == 0) counters[i] = 0u;
+  if (i < 3 * BATCH_QUEUES) counters[i * BATCH_COUNTER_LINE] = 0u;
   if (i >= ncases) return;
   Ctrl *c = cases[i].ctrl;
   if (c->t < c->tmax) {
@@ -1030,20 +1043,23 @@ __global__ void k_batch_advance(const BatchCase *cases, int ncases, unsigned int
   for (int s = 0; s < 3; ++s) c->eig[s][0][0] = 0ull;
 }
 
-hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, int nitems, unsigned int *counters,
-                              double *U, double *R, double *W, int nblocks, hipStream_t s) {
+hipError_t launch_batch_stage(int stage, const BatchCase *cases, const int *items, const BatchQueues &qs,
+                              unsigned int *counters, double *U, double *R, double *W, int nblocks, hipStream_t s) {
+  unsigned int *c = counters + (stage - 1) * BATCH_QUEUES * BATCH_COUNTER_LINE;
   switch (stage) {
-    case 1:
-      hipLaunchKernelGGL(k_batch2d<1>, dim3(nblocks), dim3(W2), 0, s, cases, items, nitems, counters + 0, U, R, W);
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_batch2d<2>, dim3(nblocks), dim3(W2), 0, s, cases, items, nitems, counters + 1, U, R, W);
-      break;
-    default:
-      hipLaunchKernelGGL(k_batch2d<3>, dim3(nblocks), dim3(W2), 0, s, cases, items, nitems, counters + 2, U, R, W);
-      break;
+    case 1: hipLaunchKernelGGL(k_batch2d<1>, dim3(nblocks), dim3(W2), 0, s, cases, items, qs, c, U, R, W); break;
+    case 2: hipLaunchKernelGGL(k_batch2d<2>, dim3(nblocks), dim3(W2), 0, s, cases, items, qs, c, U, R, W); break;
+    default: hipLaunchKernelGGL(k_batch2d<3>, dim3(nblocks), dim3(W2), 0, s, cases, items, qs, c, U, R, W); break;
   }
   return hipGetLastError();
+}
+
+int batch_stage_occupancy(int stage) {
+  int n = 0;
+  const void *k = (stage == 1) ? (const void *)k_batch2d<1> : (stage == 2) ? (const void *)k_batch2d<2>
+                                                                          : (const void *)k_batch2d<3>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, W2, 0) != hipSuccess) return 0;
+  return n;
 }
 
 hipError_t launch_batch_advance(const BatchCase *cases, int ncases, unsigned int *counters, hipStream_t s) {
