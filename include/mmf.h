@@ -269,11 +269,15 @@ int mmf_error_norms(mmf_ctx *ctx, const double *rho_exact, double *l1, double *l
  * here they are counted (per interface side, as the reference evaluates). */
 int mmf_get_positivity_counters(mmf_ctx *ctx, int64_t *neg_p, int64_t *neg_rho);
 int mmf_reset_positivity_counters(mmf_ctx *ctx);
-/* Kernel timing with HIP events on the context's stream.  While enabled,
- * every stage-kernel launch is bracketed by an event pair. */
+/* Kernel timing with HIP events on the context's stream.  on = 1: every
+ * stage-kernel launch is bracketed by an event pair (steps launched one by
+ * one); on = 2: one event pair brackets each mmf_advance call, whose steps
+ * are still replayed from graphs, and counts its three stage launches per
+ * step (the time then includes the gaps between them and, for kernels whose
+ * step ends with a separate time update, that small kernel); 0: off. */
 int mmf_enable_timing(mmf_ctx *ctx, int32_t on);
-/* total device milliseconds and launch count of the stage kernel since
- * timing was enabled (synchronises) */
+/* total device milliseconds and stage-launch count since timing was enabled
+ * (synchronises) */
 int mmf_read_timing(mmf_ctx *ctx, double *stage_ms, int64_t *stage_launches);
 
 /* ---- point functions, batched on the device (host arrays in/out) --------- */

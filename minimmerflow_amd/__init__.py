@@ -576,7 +576,9 @@ class Solver:
         self._c(lib().mmf_reset_positivity_counters(self.ctx))
 
     def enable_timing(self, on=True):
-        self._c(lib().mmf_enable_timing(self.ctx, 1 if on else 0))
+        """on: True or 1, an event pair per stage launch (steps launched one
+        by one); 2, one pair per advance call with the steps graph-replayed."""
+        self._c(lib().mmf_enable_timing(self.ctx, int(on)))
 
     def read_timing(self):
         ms, n = C.c_double(), C.c_int64()

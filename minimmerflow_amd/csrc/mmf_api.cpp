@@ -81,7 +81,7 @@ int kernel_used(const mmf_ctx *ctx) {
 // W: stage 1 writes R, stage 2 W, stage 3 U.
 int launch_stage_on(mmf_ctx *ctx, int stage, int mode, double *U, double *R, double *W) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ctx->timing) {
+  if (ctx->timing == 1) {
     HIPCHK(ctx, hipEventCreate(&e0));
     HIPCHK(ctx, hipEventCreate(&e1));
     HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
@@ -105,9 +105,9 @@ int launch_stage_on(mmf_ctx *ctx, int stage, int mode, double *U, double *R, dou
     e = mmf::launch_stage(ctx->g, stage, mode, U, R, W, R, W, U, ctx->dctrl, ctx->stream);
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "stage kernel launch");
-  if (ctx->timing) {
+  if (ctx->timing == 1) {
     HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
-    ctx->events.emplace_back(e0, e1);
+    ctx->events.push_back({e0, e1, 1});
   }
   return MMF_OK;
 }
@@ -118,13 +118,13 @@ int launch_stage_timed(mmf_ctx *ctx, int stage, int mode) {
 
 int collect_timing(mmf_ctx *ctx) {
   for (auto &p : ctx->events) {
-    HIPCHK(ctx, hipEventSynchronize(p.second));
+    HIPCHK(ctx, hipEventSynchronize(p.end));
     float ms = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms, p.first, p.second));
+    HIPCHK(ctx, hipEventElapsedTime(&ms, p.start, p.end));
     ctx->timed_ms += ms;
-    ctx->timed_launches += 1;
-    (void)hipEventDestroy(p.first);
-    (void)hipEventDestroy(p.second);
+    ctx->timed_launches += p.launches;
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.end);
   }
   ctx->events.clear();
   return MMF_OK;
@@ -274,8 +274,8 @@ int mmf_destroy(mmf_ctx *ctx) {
   mmf_output_release(ctx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto &p : ctx->events) {
-    (void)hipEventDestroy(p.first);
-    (void)hipEventDestroy(p.second);
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.end);
   }
   // the executable step graphs hold kernel nodes whose arguments point at the
   // buffers below: release them first, while every pointer is still valid
@@ -939,16 +939,11 @@ void mmf_drop_step_graphs(mmf_ctx *ctx) {
   ctx->graph_key.clear();
 }
 
-// Steps are replayed from captured HIP graphs (GRAPH_STEPS steps per replay,
-// then single steps), re-captured whenever the launch configuration changed;
-// with kernel timing on they are launched one by one.
-int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
-  if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
-  int rct = ensure_tiles(ctx);
-  if (rct != MMF_OK) return rct;
-  int rc0 = ensure_lam(ctx);
-  if (rc0 != MMF_OK) return rc0;
-  if (ctx->timing || !ctx->use_graphs || nsteps == 0) return enqueue_steps(ctx, nsteps);
+namespace {
+
+// Steps replayed from captured HIP graphs (GRAPH_STEPS steps per replay, then
+// single steps), re-captured whenever the launch configuration changed.
+int replay_steps(mmf_ctx *ctx, int64_t nsteps) {
   std::vector<char> key = launch_key(ctx);
   if (key != ctx->graph_key) {
     mmf_drop_step_graphs(ctx);
@@ -967,6 +962,32 @@ int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
     nsteps -= slot ? GRAPH_STEPS : 1;
   }
   return MMF_OK;
+}
+
+}  // namespace
+
+// Steps are replayed from graphs, or launched one by one with per-launch
+// kernel timing on; region timing brackets the whole call with one event pair.
+int mmf_advance(mmf_ctx *ctx, int64_t nsteps) {
+  if (!ctx || nsteps < 0) return MMF_ERR_INVALID;
+  int rct = ensure_tiles(ctx);
+  if (rct != MMF_OK) return rct;
+  int rc0 = ensure_lam(ctx);
+  if (rc0 != MMF_OK) return rc0;
+  hipEvent_t r0 = nullptr, r1 = nullptr;
+  if (ctx->timing == 2 && nsteps > 0) {
+    HIPCHK(ctx, hipEventCreate(&r0));
+    if (hipEventCreate(&r1) != hipSuccess) {
+      (void)hipEventDestroy(r0);
+      return fail(ctx, MMF_ERR_HIP, "timing event");
+    }
+    ctx->events.push_back({r0, r1, 3 * nsteps});
+    HIPCHK(ctx, hipEventRecord(r0, ctx->stream));
+  }
+  const int rc = (ctx->timing == 1 || !ctx->use_graphs || nsteps == 0) ? enqueue_steps(ctx, nsteps)
+                                                                        : replay_steps(ctx, nsteps);
+  if (r1) HIPCHK(ctx, hipEventRecord(r1, ctx->stream));
+  return rc;
 }
 
 int mmf_set_step_graphs(mmf_ctx *ctx, int32_t on) {
@@ -1075,10 +1096,10 @@ int mmf_reset_positivity_counters(mmf_ctx *ctx) {
 }
 
 int mmf_enable_timing(mmf_ctx *ctx, int32_t on) {
-  if (!ctx) return MMF_ERR_INVALID;
+  if (!ctx || on < 0 || on > 2) return MMF_ERR_INVALID;
   int rc = collect_timing(ctx);
   if (rc != MMF_OK) return rc;
-  ctx->timing = on != 0;
+  ctx->timing = on;
   ctx->timed_ms = 0.0;
   ctx->timed_launches = 0;
   return MMF_OK;

@@ -34,8 +34,14 @@ struct mmf_ctx {
   int kernel = mmf::KERNEL_AUTO;
   mmf::MarchShape shape{8, 32};
   std::string err;
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  // kernel timing: 0 off, 1 an event pair around every stage launch, 2 one
+  // pair around each mmf_advance call (steps still replayed from graphs)
+  int timing = 0;
+  struct TimedSpan {
+    hipEvent_t start, end;
+    int64_t launches;  // stage launches between the two events
+  };
+  std::vector<TimedSpan> events;
   double timed_ms = 0.0;
   int64_t timed_launches = 0;
   // learned face reconstruction (2D, order 2): active when learn_on

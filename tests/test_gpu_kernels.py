@@ -268,3 +268,28 @@ def test_bodies_changed_between_graph_replayed_steps(mmf):
         assert np.array_equal(fa, fb)
         assert ta == tb
         assert np.array_equal(a, b)
+
+
+def test_timing_modes_count_launches_and_keep_results(mmf):
+    """Per-launch event pairs (steps launched one by one) and one pair per
+    advance call (steps replayed from graphs) count three stage launches per
+    step and leave the results as with timing off."""
+    s, U = build(mmf, 3, [64, 40, 36], [0] * 6)
+    s.set_kernel(mmf.KERNEL_MARCH)
+    out = []
+    for mode in (0, 1, 2):
+        s.set_state(U)
+        s.set_time(0.0, 10.0, 0.5)
+        s.enable_timing(mode)
+        s.advance(5)
+        s.advance(2)
+        ms, n = s.read_timing()
+        s.enable_timing(0)
+        if mode:
+            assert n == 21 and ms > 0.0, (mode, n, ms)
+        out.append((s.get_state(), s.time()))
+    for st, tm in out[1:]:
+        assert np.array_equal(st, out[0][0]) and tm[:3] == out[0][1][:3]
+    with pytest.raises(mmf.MMFError):
+        s.enable_timing(3)
+    s.close()
