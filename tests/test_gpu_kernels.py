@@ -242,13 +242,16 @@ def test_tile_kernel_refused_where_it_does_not_apply(mmf):
 def test_bodies_changed_between_graph_replayed_steps(mmf):
     """Flags changed on a live context after graph-captured steps: a body moved
     (the tile table rewritten under the same pointer), removed (no flags at all)
-    and set again; every phase bit-identical to the gather kernel."""
+    and set again; every phase bit-identical to the gather kernel, with one
+    (tile, chunk) per block and with the planes split over 7 blocks (the
+    segment table rebuilt from the new flags)."""
     info = np.zeros((6, 5))
     info[:] = [1.0, 0.3, -0.1, 0.05, 1.0]
     bodies = [{"boxes": [[-0.4, -0.3, -0.5, 0.1, 0.35, 0.2]]}, {"spheres": [[0.2, 0.1, -0.1, 0.3]]}, None,
               {"boxes": [[0.5, -0.9, -0.9, 0.8, -0.5, 0.0]]}]
     out = {}
-    for kind in (mmf.KERNEL_GATHER, mmf.KERNEL_MARCH):
+    for variant in ("gather", "march", "march segments"):
+        kind = mmf.KERNEL_GATHER if variant == "gather" else mmf.KERNEL_MARCH
         s, U = build(mmf, 3, [64, 24, 20], [3, 0, 0, 3, 0, 3], info)
         s.set_state(U)
         s.set_time(0.0, 10.0, 0.5)
@@ -259,15 +262,19 @@ def test_bodies_changed_between_graph_replayed_steps(mmf):
             else:
                 s.set_bodies(**b)
             s.set_kernel(kind, 8, 8)
+            if variant == "march segments":
+                s.set_march_blocks(7)
+                assert s.kernel()[2] == 0
             assert s.kernel()[0] == kind
             s.advance(17)  # one 16-step graph and one 1-step graph
             res.append((s.get_state(), s.time()[:3], s.cell_flags()))
-        out[kind] = res
+        out[variant] = res
         s.close()
-    for (a, ta, fa), (b, tb, fb) in zip(out[mmf.KERNEL_GATHER], out[mmf.KERNEL_MARCH]):
-        assert np.array_equal(fa, fb)
-        assert ta == tb
-        assert np.array_equal(a, b)
+    for variant in ("march", "march segments"):
+        for (a, ta, fa), (b, tb, fb) in zip(out["gather"], out[variant]):
+            assert np.array_equal(fa, fb), variant
+            assert ta == tb, variant
+            assert np.array_equal(a, b), variant
 
 
 def test_timing_modes_count_launches_and_keep_results(mmf):
