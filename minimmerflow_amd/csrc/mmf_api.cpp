@@ -24,6 +24,13 @@
 
 namespace {
 
+// Every fluid cell is solved (no ghost cells): the marching kernel's early dt applies.
+int32_t ghost_free_of(const std::vector<uint8_t> &flags) {
+  for (uint8_t v : flags)
+    if ((v & MMF_CELL_FLUID) && !(v & MMF_CELL_SOLVED)) return 0;
+  return 1;
+}
+
 int fail(mmf_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->err = msg;
   return code;
@@ -189,6 +196,7 @@ int grid_from_desc(const mmf_grid_desc *d, GridDev &g) {
   }
   for (int a = 0; a < 3; ++a) g.periodic[a] = (a < d->dim) && d->side_bc[2 * a] == MMF_BC_PERIODIC;
   g.flags = nullptr;
+  g.ghost_free = 1;
   int e2 = 0;
   g.pow2V = std::frexp(g.V, &e2) == 0.5 ? 1 : 0;
   g.invV = 1.0 / g.V;
@@ -402,12 +410,14 @@ int mmf_set_cell_flags(mmf_ctx *ctx, const uint8_t *fluid, const uint8_t *solved
     if (ctx->dflags) (void)hipFree(ctx->dflags);
     ctx->dflags = nullptr;
     ctx->g.flags = nullptr;
+    ctx->g.ghost_free = 1;
     ctx->hflags.clear();
     return MMF_OK;
   }
   if (!ctx->dflags) HIPCHK(ctx, hipMalloc(&ctx->dflags, (size_t)N));
   HIPCHK(ctx, hipMemcpy(ctx->dflags, f.data(), (size_t)N, hipMemcpyHostToDevice));
   ctx->g.flags = ctx->dflags;
+  ctx->g.ghost_free = ghost_free_of(f);
   ctx->hflags.swap(f);
   return MMF_OK;
 }
@@ -546,10 +556,12 @@ int mmf_set_bodies(mmf_ctx *ctx, const double *boxes, int32_t nb, const double *
     (void)hipFree(dfl);
     ctx->dflags = nullptr;
     ctx->g.flags = nullptr;
+    ctx->g.ghost_free = 1;
     ctx->hflags.clear();
   } else {
     ctx->dflags = dfl;
     ctx->g.flags = dfl;
+    ctx->g.ghost_free = ghost_free_of(h);
     ctx->hflags.swap(h);
   }
   return MMF_OK;

@@ -23,6 +23,7 @@ struct GridDev {
   int32_t periodic[3];
   double info[6][5];   // Dirichlet primitives per side
   const uint8_t *flags;  // MMF_CELL_* bits per cell, or nullptr (all fluid+solved)
+  int32_t ghost_free;    // with flags: every fluid cell is solved (no ghost cells)
   double invV;         // 1/V, used only when V is a power of two (x/V == x*invV exactly)
   int32_t pow2V;
   int32_t pow2A;       // A is a power of two: A*(0.5*x) == (0.5*A)*x exactly
@@ -166,7 +167,8 @@ bool build_march_segments(const GridDev &g, const MarchShape &shape, const uint8
 // num_cus CUs and every share stays within a few tiles.
 void balance_march_shape(const GridDev &g, int num_cus, MarchShape &shape);
 int64_t march_blocks(const GridDev &g, const MarchShape &shape);
-// The marching kernel takes dt from Ctrl::lam_now (early dt) on this grid.
+// The marching kernel takes dt from Ctrl::lam_now (early dt) on this grid:
+// 3D order 1 without flags, or with flags where every fluid cell is solved.
 bool march_early(const GridDev &g);
 // Ctrl::lam_now from state U (early dt before a step sequence).
 hipError_t launch_lam_init(const GridDev &g, const double *U, Ctrl *ctrl, hipStream_t stream);

@@ -201,34 +201,50 @@ __global__ __launch_bounds__(BLOCK) void k_stage(GridDev g, StageIn<STAGE> in, c
 // t += dt after stage 3 (main.cpp:505-506); resets the stage maxima.
 __global__ void k_advance(Ctrl *c) { advance_ctrl_wave(c, threadIdx.x); }
 
-// The early maximum of state U: max over cells and axes of |u_a| + c, and
-// over each Dirichlet border's virtual state (euler.cpp:366-376) -- on a grid
-// without a body or a reflected border, what the stage-1 faces of a step
-// starting from U give (each face takes the larger of its two sides, and
-// every cell has faces along every axis).  Ctrl::lam_now is zero beforehand.
+// The early maximum of state U: what the stage-1 faces of a step starting
+// from U give on a grid whose unsolved cells are all solid (every face with a
+// solved side takes the larger of its two sides' |u_a| + c, euler.cpp:232-234):
+// over the solved cells, every axis (each has faces along every axis); along
+// an axis with a wall face (a non-fluid neighbour) or a reflecting / wall
+// border, the reflected virtual state's (euler.cpp:318-356); at a Dirichlet
+// border, the border state's (euler.cpp:366-376).  Ctrl::lam_now is zero
+// beforehand.
 __global__ __launch_bounds__(256) void k_lam_init(GridDev g, const double *U, Ctrl *ctrl) {
   double m = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += stride) {
+    if (g.flags && !(g.flags[i] & CELL_SOLVED)) continue;
     double c[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) c[k] = U[k * g.fs + i];
     Tup T;
     make_tup(c, T);
+    const int64_t q[3] = {i % g.n[0], (i / g.n[0]) % g.n[1], i / (g.n[0] * g.n[1])};
     for (int a = 0; a < g.dim; ++a) {
-      const double l = fabs(T.vel[a]) + T.a;
-      m = lam_acc(m, l);
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x < 2 * g.dim) {
-    const int side = threadIdx.x, a = side / 2;
-    if (g.bc[side] == BC_DIRICHLET && !g.periodic[a]) {
-      double v[NF];
-      Tup V;
-      prim2cons(g.info[side], v);
-      make_tup(v, V);
-      const double l = fabs(V.vel[a]) + V.a;
-      m = lam_acc(m, l);
+      m = lam_acc(m, fabs(T.vel[a]) + T.a);
+      bool refl = false;
+      for (int d = 0; d < 2; ++d) {
+        const int side = 2 * a + d;
+        int64_t nb;
+        if (neighbour(g, q, a, d ? 1 : -1, nb)) {
+          refl = refl || (g.flags && !(g.flags[nb] & CELL_FLUID));
+        } else if (g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) {
+          refl = true;
+        } else if (g.bc[side] == BC_DIRICHLET) {
+          double v[NF];
+          Tup V;
+          prim2cons(g.info[side], v);
+          make_tup(v, V);
+          m = lam_acc(m, fabs(V.vel[a]) + V.a);
+        }
+      }
+      if (refl) {
+        double v[NF];
+        Tup V;
+        reflect_axis(c, a, v);
+        make_tup(v, V);
+        m = lam_acc(m, fabs(V.vel[a]) + V.a);
+      }
     }
   }
   __shared__ unsigned long long red[4];

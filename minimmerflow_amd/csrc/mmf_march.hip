@@ -306,8 +306,53 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   gp.pow2V = P2;
   const double A = g.A, halfA = g.halfA;
 
+  // Early dt with a body, a reflecting or wall border or a Dirichlet border
+  // on a grid with flags: the next step's stage-1 faces of this solved cell
+  // also take, along each axis with a wall face (a non-fluid neighbour) or
+  // a reflecting / wall border, the reflected state's |u_a| + c (the virtual
+  // state of virtual_face, euler.cpp:318-356), and at a Dirichlet border on a
+  // grid with flags the border state's (a grid without flags adds those once
+  // per block: every border cell is solved there).
+  auto virtual_max_eig = [&](const GridDev &gg, const double *c, const uint8_t *LFp, int mm, uint8_t fbelow,
+                             uint8_t fabove, bool above) {
+    double l = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      bool refl = false;
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const int side = 2 * a + d;
+        bool has;
+        uint8_t fnb = CELL_FLUID | CELL_SOLVED;
+        if (a == M) {
+          has = d ? above : (mm > 0 || gg.periodic[M]);
+          fnb = d ? fabove : fbelow;
+        } else {
+          has = (a == 0) ? (d ? has_xp : has_xm) : (d ? has_yp : has_ym);
+          if (MASK && has) fnb = LFp[myslot + (d ? 1 : -1) * ((a == 0) ? 1 : RW)];
+        }
+        if (has) {
+          refl = refl || (MASK && !(fnb & CELL_FLUID));
+        } else {
+          refl = refl || (REFL && (gg.bc[side] == BC_REFLECTING || gg.bc[side] == BC_WALL));
+          if (gg.flags != nullptr && gg.bc[side] == BC_DIRICHLET)
+            l = lam_acc(l, fabs(bct[side * NTUP + 5 + a]) + bct[side * NTUP + 9]);
+        }
+      }
+      if (refl) {
+        double v[NF];
+        Tup V;
+        reflect_axis(c, a, v);
+        make_tup(v, V);
+        l = lam_acc(l, fabs(V.vel[a]) + V.a);
+      }
+    }
+    return l;
+  };
+
   Tup Tc;  // tuple of this column's cell in plane m
   uint8_t fc = CELL_FLUID | CELL_SOLVED;
+  uint8_t fb = CELL_FLUID | CELL_SOLVED;  // the cell below plane m (MASK)
   bool bad_c = false;   // Tc has a negative p or rho
   double AFlo[NF];      // A F of the face below plane m
   double Ucur[NF];      // U of the cell in plane m (stage 2: carried from its load)
@@ -359,6 +404,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
         src.load(gp, dt, pidx, Xp);
         make_tup(Xp, Tp);
         const uint8_t fp = cell_flags<MASK>(g, pidx);
+        fb = fp;
         // counted here: the block below does not count its chunk's top face
         if ((fp | fc) & CELL_SOLVED) {
           double l;
@@ -529,7 +575,10 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
           nu[k] = w1 ? Tc.c[k] + div_vol(gp, dt * rhs) : rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt);
           outf.put(k, idx, nu[k]);
         }
-        if (EARLY && STAGE == 3) lam_nx = lam_acc(lam_nx, cell_max_eig<DIM>(nu));
+        if (EARLY && STAGE == 3) {
+          lam_nx = lam_acc(lam_nx, cell_max_eig<DIM>(nu));
+          if (MASK || REFL) lam_nx = lam_acc(lam_nx, virtual_max_eig(gp, nu, LF, m, fb, fn, has_next));
+        }
       } else if (STAGE != 3) {
 #pragma unroll
         for (int k = 0; k < NF; ++k)
@@ -540,6 +589,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
 #pragma unroll
       for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
       Tc = Tn;
+      fb = fc;
       fc = fn;
       bad_c = bad_n;
       if (RB) {
@@ -557,7 +607,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
 // each, so one block's plane barrier and division chains overlap the other's
 // work) or 32 x 16 (eight waves, a halo ring of 19% of the tile's cells
 // instead of 28% for 64 x 8).
-template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool SEGS>
+template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool SEGS, bool EARLY>
 __global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
                                                      Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
   constexpr int PL = (TY + 2) * (TX + 2);
@@ -565,7 +615,6 @@ __global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> i
   __shared__ double lds[2 * NTUP * PL];
   __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
   __shared__ double bct[6 * NTUP];
-  constexpr bool EARLY = !MASK && !REFL;
   constexpr int NT = TX * TY;
   if (mode == MODE_STEP && !(ctrl->t < ctrl->tmax)) return;  // main.cpp:377 loop guard
   const int tid = threadIdx.y * TX + threadIdx.x;
@@ -613,7 +662,9 @@ __global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> i
   neg.flush(ctrl, tid);
   block_max_to<NT>(ctrl, (mode == MODE_RHS_ONLY) ? 3 : STAGE - 1, lam_max, tid);
   if (EARLY && STAGE == 3 && mode == MODE_STEP) {
-    if (tid == 0 && any) {  // a Dirichlet border's virtual state enters the next maximum too
+    // a Dirichlet border's virtual state enters the next maximum too (on a
+    // grid with flags the border cells add it, where they are solved)
+    if (tid == 0 && any && g.flags == nullptr) {
 #pragma unroll
       for (int side = 0; side < 2 * DIM; ++side)
         if (g.bc[side] == BC_DIRICHLET && !g.periodic[side / 2]) {
@@ -627,7 +678,7 @@ __global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> i
   if (STAGE == 3 && mode == MODE_STEP) close_step<EARLY>(ctrl, tid);
 }
 
-template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL>
+template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, bool EARLY>
 static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mode, const double *U, const double *R,
                                  const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl, hipStream_t s) {
   StageIn<STAGE> in;
@@ -635,9 +686,7 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   in.dt = 0.0;
   in.U = U;
   in.R = R;
-  // with early dt (no body, no reflected border) stage 2 reads the W1 that
-  // stage 1 wrote into R's buffer
-  constexpr bool EARLY = !MASK && !REFL;
+  // with early dt stage 2 reads the W1 that stage 1 wrote into R's buffer
   in.X = (STAGE == 3) ? W : (STAGE == 2 && EARLY) ? R : U;
   double *out = (STAGE == 1) ? oR : (STAGE == 2) ? oW : oU;
   const int M = DIM - 1;
@@ -648,47 +697,53 @@ static hipError_t launch_march_t(const GridDev &g, const MarchShape &sh, int mod
   // one (tile, chunk) per block
   if constexpr (DIM == 3 && TX == 64 && TY == 8) {
     if (sh.seg_blocks > 0 && g.march_tiles != nullptr) {
-      hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL, true>), dim3((unsigned)sh.seg_blocks), block,
-                         0, s, g, in, U, out, ctrl, mode, 0, ntx, nty);
+      hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL, true, EARLY>), dim3((unsigned)sh.seg_blocks),
+                         block, 0, s, g, in, U, out, ctrl, mode, 0, ntx, nty);
       return hipGetLastError();
     }
   }
   const int nch = (int)((g.n[M] + sh.chunk - 1) / sh.chunk);
-  hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL, false>), dim3((unsigned)(ntx * nty * nch)), block,
-                     0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx, nty);
+  hipLaunchKernelGGL((k_march<DIM, TX, TY, STAGE, MASK, P2, REFL, false, EARLY>), dim3((unsigned)(ntx * nty * nch)),
+                     block, 0, s, g, in, U, out, ctrl, mode, sh.chunk, ntx, nty);
   return hipGetLastError();
 }
 
-template <int DIM, int TX, int TY, bool MASK, bool P2, bool REFL>
+template <int DIM, int TX, int TY, bool MASK, bool P2, bool REFL, bool EARLY>
 static hipError_t launch_march_p(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
   switch (stage) {
-    case 1: return launch_march_t<DIM, TX, TY, 1, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    case 2: return launch_march_t<DIM, TX, TY, 2, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
-    default: return launch_march_t<DIM, TX, TY, 3, MASK, P2, REFL>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 1: return launch_march_t<DIM, TX, TY, 1, MASK, P2, REFL, EARLY>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    case 2: return launch_march_t<DIM, TX, TY, 2, MASK, P2, REFL, EARLY>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
+    default: return launch_march_t<DIM, TX, TY, 3, MASK, P2, REFL, EARLY>(g, sh, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
 }
 
+// Instances: without flags, early dt always (REFL: a reflecting or wall
+// border); with flags (MASK, which implies REFL), early dt where every fluid
+// cell is solved, else stage 1 writes R and stage 2 rebuilds W1.
 template <int DIM, int TX, int TY, bool MASK>
 static hipError_t launch_march_s(const GridDev &g, const MarchShape &sh, int stage, int mode, const double *U,
                                  const double *R, const double *W, double *oR, double *oW, double *oU, Ctrl *ctrl,
                                  hipStream_t s) {
-  // a body (walls) or a reflecting / wall border needs the reflected virtual state
-  bool refl = MASK;
-  for (int side = 0; side < 2 * DIM; ++side)
-    refl = refl || ((g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) && !g.periodic[side / 2]);
-  if (refl) {
-    if (g.pow2A && g.pow2V)
-      return launch_march_p<DIM, TX, TY, MASK, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-    return launch_march_p<DIM, TX, TY, MASK, false, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  const bool p2 = g.pow2A && g.pow2V;
+  if constexpr (MASK) {
+    if (march_early(g))
+      return p2 ? launch_march_p<DIM, TX, TY, true, true, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                : launch_march_p<DIM, TX, TY, true, false, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return p2 ? launch_march_p<DIM, TX, TY, true, true, true, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+              : launch_march_p<DIM, TX, TY, true, false, true, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+  } else {
+    // a reflecting / wall border needs the reflected virtual state
+    bool refl = false;
+    for (int side = 0; side < 2 * DIM; ++side)
+      refl = refl || ((g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) && !g.periodic[side / 2]);
+    if (refl)
+      return p2 ? launch_march_p<DIM, TX, TY, false, true, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+                : launch_march_p<DIM, TX, TY, false, false, true, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
+    return p2 ? launch_march_p<DIM, TX, TY, false, true, false, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s)
+              : launch_march_p<DIM, TX, TY, false, false, false, true>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
   }
-  if constexpr (!MASK) {
-    if (g.pow2A && g.pow2V)
-      return launch_march_p<DIM, TX, TY, false, true, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-    return launch_march_p<DIM, TX, TY, false, false, false>(g, sh, stage, mode, U, R, W, oR, oW, oU, ctrl, s);
-  }
-  return hipErrorInvalidValue;  // not reached (MASK implies refl)
 }
 
 int64_t march_blocks(const GridDev &g, const MarchShape &sh) {
@@ -875,10 +930,7 @@ bool march_shape_supported(const GridDev &g, const MarchShape &sh) {
 }
 
 bool march_early(const GridDev &g) {
-  if (g.dim != 3 || g.order != 1 || g.flags != nullptr) return false;
-  for (int side = 0; side < 6; ++side)
-    if ((g.bc[side] == BC_REFLECTING || g.bc[side] == BC_WALL) && !g.periodic[side / 2]) return false;
-  return true;
+  return g.dim == 3 && g.order == 1 && (g.flags == nullptr || g.ghost_free);
 }
 
 // Longest chunk (fewest boundary planes re-read) that still gives every CU a
