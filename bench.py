@@ -21,8 +21,9 @@ steps (cell-updates per second per RK stage).
 Prints ONE JSON line on stdout.  The roofline entry is for the stage kernel:
 achieved = algorithmic bytes per launch (80/120/120 B per cell for stages
 1/2/3, mean 106.67 B; +1 B per cell for the body mask) / average launch
-duration from HIP events recorded on the solver's stream around every stage
-launch of the timed steps.
+duration: one HIP event pair recorded on the solver's stream around the
+timed steps (replayed from the step graphs, as in any run) divided by their
+stage launches, so the short gaps between launches count as kernel time.
 """
 import argparse
 import json
