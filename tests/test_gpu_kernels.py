@@ -300,3 +300,34 @@ def test_timing_modes_count_launches_and_keep_results(mmf):
     with pytest.raises(mmf.MMFError):
         s.enable_timing(3)
     s.close()
+
+
+@pytest.mark.parametrize("covered", [True, False])
+def test_early_dt_dirichlet_side_behind_a_body(mmf, covered):
+    """A Dirichlet side whose border cells are all solid has no face in the
+    stage maximum (euler.cpp:181-183): the marching kernel's early dt on a grid
+    with flags leaves that side's state out, as the gather kernel does; with
+    the side open its state (faster than every cell) sets the first dt."""
+    info = np.zeros((6, 5))
+    info[0] = [1.0, 4.0, 0.0, 0.0, 1.0]  # |u| + c = 4 + sqrt(1.4): above every cell's
+    # the first cell column (centroid x = -0.984) solid, or a box away from the side
+    box = [-1.0, -1.0, -1.0, -0.98, 1.0, 1.0] if covered else [0.2, -0.9, -0.9, 0.4, -0.7, -0.7]
+    h = 2.0 / 64
+    dt_inflow = 0.9 * 0.5 * h / (4.0 + np.sqrt(1.4))
+    out = []
+    for kind in (mmf.KERNEL_GATHER, mmf.KERNEL_MARCH):
+        s, U = build(mmf, 3, [64, 16, 16], [3, 0, 0, 0, 0, 0], info, {"boxes": [box]})
+        s.set_kernel(kind)
+        assert s.kernel()[0] == kind
+        s.set_state(U)
+        s.set_time(0.0, 10.0, 0.5)
+        s.advance(1)
+        dt1 = s.time()[1]
+        assert abs(dt1 - dt_inflow) <= 1e-12 * dt_inflow if not covered else dt1 > 1.01 * dt_inflow, (kind, dt1)
+        s.advance(2)
+        out.append((s.get_state(), s.time(), dt1))
+        s.close()
+    (Ug, tg, d1g), (Um, tm, d1m) = out
+    assert d1g == d1m
+    assert tg[:3] == tm[:3] and np.array_equal(tg[3], tm[3]), (tg, tm)
+    assert np.array_equal(Ug, Um)
