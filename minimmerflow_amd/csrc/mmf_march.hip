@@ -304,7 +304,13 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   GridDev gp = g;
   gp.pow2A = P2;
   gp.pow2V = P2;
-  const double A = g.A, halfA = g.halfA;
+  // P2: the faces hand over their unscaled sums X (halfA = 1, the multiply
+  // folds away) and the cell's total is scaled by 0.5 A once at the update.
+  // A power-of-two factor commutes with every rounding of the sum, so this is
+  // the double of sum(0.5 A X) unless a partial sum leaves the normal range.
+  if (P2) gp.halfA = 1.0;
+  const double A = g.A, halfA = P2 ? 1.0 : g.halfA;
+  const double rhs_scale = g.halfA;
 
   // Early dt with a body, a reflecting or wall border or a Dirichlet border
   // on a grid with flags: the next step's stage-1 faces of this solved cell
@@ -571,7 +577,8 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
         double nu[NF];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-          const double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
+          double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
+          if (P2) rhs = rhs_scale * rhs;
           nu[k] = w1 ? Tc.c[k] + div_vol(gp, dt * rhs) : rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt);
           outf.put(k, idx, nu[k]);
         }
