@@ -115,6 +115,10 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
   GridDev g = g0;
   g.pow2A = P2;
   g.pow2V = P2;
+  // P2: faces hand over unscaled sums, the cell's total is scaled by 0.5 A
+  // once (the same double: a power-of-two factor commutes with the roundings)
+  if (P2) g.halfA = 1.0;
+  const double rhs_scale = g0.halfA;
   const int lane = threadIdx.x;
   const int nx = (int)g.n[0];
   const int nm = (int)g.n[1];
@@ -363,7 +367,8 @@ __device__ double march2d_item(const GridDev &g0, const In2<STAGE, P2> &in, cons
         const bool wrap_top = top_wrap_here && m == nm - 1;
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-          const double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
+          double rhs = wrap_top ? (acc[k] - AFhi[k]) + AFlo[k] : (acc[k] + AFlo[k]) - AFhi[k];
+          if (P2) rhs = rhs_scale * rhs;
           outb[k * fs + idx] = rk_update<STAGE>(g, Ucur[k], Tc.c[k], rhs, dt);
         }
       } else if (STAGE != 3) {
@@ -403,6 +408,8 @@ __device__ double march2d_item_flat(const GridDev &g0, const In2<STAGE, P2> &in,
   GridDev g = g0;
   g.pow2A = P2;
   g.pow2V = P2;
+  if (P2) g.halfA = 1.0;  // unscaled face sums, as in march2d_item
+  const double rhs_scale = g0.halfA;
   const int lane = threadIdx.x;
   const int nx = (int)g.n[0];
   const int nm = (int)g.n[1];
@@ -563,7 +570,8 @@ __device__ double march2d_item_flat(const GridDev &g0, const In2<STAGE, P2> &in,
       double acc = 0.0;  // first two contributions from zero: their order does not change the bits
       acc += AFx[k];
       acc -= AFup[k];
-      const double rhs = wrap_top ? (acc - AFhi[k]) + AFlo[k] : (acc + AFlo[k]) - AFhi[k];
+      double rhs = wrap_top ? (acc - AFhi[k]) + AFlo[k] : (acc + AFlo[k]) - AFhi[k];
+      if (P2) rhs = rhs_scale * rhs;
       nu[k] = rk_update<STAGE>(g, Ucur[k], Tc.c[k], rhs, dt);
     }
     if (valid) {
