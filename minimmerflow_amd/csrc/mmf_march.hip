@@ -219,6 +219,9 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   constexpr int M = DIM - 1;  // march axis
   constexpr int RW = TX + 2;
   constexpr int PL = (TY + 2) * RW;
+  // field stride of a tuple plane in LDS: the tile and its halo ring, then one
+  // slot per border side holding that side's Dirichlet tuple (both buffers)
+  constexpr int PS = PL + 8;
   // Early dt (no body, no reflected border): the step's dt is known before
   // stage 1 (Ctrl::lam_now), stage 1 writes W1 = U + dt R / V itself, stage 2
   // reads W1 like stage 3 reads W, and stage 3 gathers the next step's
@@ -229,6 +232,16 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   const double dt = EARLY ? ((mode == MODE_STEP) ? step_dt_early(ctrl) : 0.0) : ((STAGE >= 2) ? step_dt(ctrl) : 0.0);
   dirichlet_tuples(g, bct, threadIdx.x + TX * threadIdx.y);
   __syncthreads();
+  if constexpr (DIM == 3 && !MASK && !REFL) {
+    // the Dirichlet tuples as the border slots of both buffers (the flat
+    // plane pass below loads a border face's virtual state like a neighbour)
+    const int t = threadIdx.x + TX * threadIdx.y;
+    if (t < 6 * NTUP) {
+      const int side = t / NTUP, k = t % NTUP;
+      lds[k * PS + PL + side] = bct[side * NTUP + k];
+      lds[NTUP * PS + k * PS + PL + side] = bct[side * NTUP + k];
+    }
+  }
 
   Source<RB> src;
   src.X.set(in.X, g.fs);
@@ -380,7 +393,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
       make_tup(X, Tc);
       bad_c = tup_negative(Tc);
       fc = cell_flags<MASK>(g, idx0);
-      tup_store<PL>(lds, myslot, Tc);
+      tup_store<PS>(lds, myslot, Tc);
       if (MASK) lfl[myslot] = fc;
     }
     if (hslot >= 0) {
@@ -388,7 +401,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
       Tup Th;
       src.load(gp, dt, hcol + (uint32_t)m0 * sm, Xh);
       make_tup(Xh, Th);
-      tup_store<PL>(lds, hslot, Th);
+      tup_store<PS>(lds, hslot, Th);
       if (MASK) lfl[hslot] = g.flags[hcol + (uint32_t)m0 * sm];
     }
     if (valid) {
@@ -443,11 +456,102 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
   __syncthreads();
 
   int buf = 0;
-  for (int m = m0; m < m1; ++m) {
+  int m = m0;
+  if constexpr (DIM == 3 && !MASK && !REFL && !RB) {
+    // Flat pass: a full tile whose x and y borders are free flow or Dirichlet
+    // (no periodic wrap), every plane but the chunk's last.  Every lane is a
+    // cell, every face of the plane is evaluated from a tuple in LDS (a
+    // border face's virtual state sits in a slot of its own: the cell's own
+    // tuple for free flow, the side's Dirichlet tuple), the plane above is in
+    // the chunk, and the negative-state counts are added without branches, so
+    // the pass (next plane's tuple, the in-plane faces, the face above, the
+    // update) is one straight block whose division chains and face fluxes the
+    // compiler interleaves.  The operations and their order are those of the
+    // general pass below, so the results are the same doubles and counts.
+    bool flat = w == TX && hgt == TY && !g.periodic[0] && !g.periodic[1];
+#pragma unroll
+    for (int sd = 0; sd < 4; ++sd) flat = flat && (g.bc[sd] == BC_FREE_FLOW || g.bc[sd] == BC_DIRICHLET);
+    if (flat) {
+      const int s_nb[4] = {has_xm ? myslot - 1 : (g.bc[0] == BC_DIRICHLET ? PL + 0 : myslot),
+                           has_xp ? myslot + 1 : (g.bc[1] == BC_DIRICHLET ? PL + 1 : myslot),
+                           has_ym ? myslot - RW : (g.bc[2] == BC_DIRICHLET ? PL + 2 : myslot),
+                           has_yp ? myslot + RW : (g.bc[3] == BC_DIRICHLET ? PL + 3 : myslot)};
+      const bool border_m[2] = {!has_xm, !has_ym};
+      for (; m + 1 < m1; ++m) {
+        const uint32_t idx = col + (uint32_t)m * sm;
+        const double *L = lds + buf * (NTUP * PS);
+        double *Ln = lds + (buf ^ 1) * (NTUP * PS);
+        if (STAGE >= 2) {
+#pragma unroll
+          for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
+        }
+        // halo ring of plane m + 1 (waves 0-2), and its loads of plane m + 2
+        if (hslot >= 0) {
+          double Xh[NF];
+          Tup Th;
+          ph.state(gp, dt, Xh);
+          make_tup(Xh, Th);
+          tup_store<PS>(Ln, hslot, Th);
+          if (m + 2 < m1) ph.issue(src, hcol + (uint32_t)(m + 2) * sm);
+        }
+        // plane m + 1's tuple, then the loads of plane m + 2 (plane m + 1 again
+        // where no plane m + 2 is needed: an in-bounds load left unused)
+        Tup Tn;
+        {
+          double X[NF];
+          pf.state(gp, dt, X);
+          make_tup(X, Tn);
+          tup_store<PS>(Ln, myslot, Tn);
+          const uint32_t p2 = (m + 2 < m1) ? (uint32_t)(m + 2) : (have_beyond ? beyond : (uint32_t)(m + 1));
+          pf.issue(src, col + p2 * sm);
+        }
+        // in-plane faces, reference order
+        double acc[NF] = {0., 0., 0., 0., 0.};
+#pragma unroll
+        for (int a = 0; a < DIM - 1; ++a) {
+          double AFm[NF], AFp[NF];
+          Tup Nm, Np;
+          tup_load<PS>(L, s_nb[2 * a], Nm);
+          tup_load<PS>(L, s_nb[2 * a + 1], Np);
+          const double lm = face_flux_area(Nm, Tc, a, pow2A, A, halfA, AFm);
+          neg.add_if(border_m[a], Nm, Tc);
+          const double lp = face_flux_area(Tc, Np, a, pow2A, A, halfA, AFp);
+          neg.add(Tc, Np);
+          lam_max = lam_acc(lam_max, lm);
+          lam_max = lam_acc(lam_max, lp);
+#pragma unroll
+          for (int k = 0; k < NF; ++k) { acc[k] += AFm[k]; acc[k] -= AFp[k]; }
+        }
+        // the face above, which completes the cell
+        double AFhi[NF];
+        {
+          const double l = face_flux_area(Tc, Tn, M, pow2A, A, halfA, AFhi);
+          neg.add(Tc, Tn);
+          lam_max = lam_acc(lam_max, l);
+        }
+        double nu[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+          double rhs = (acc[k] + AFlo[k]) - AFhi[k];
+          if (P2) rhs = rhs_scale * rhs;
+          nu[k] = w1 ? Tc.c[k] + div_vol(gp, dt * rhs) : rk_update<STAGE>(gp, Ucur[k], Tc.c[k], rhs, dt);
+          outf.put(k, idx, nu[k]);
+        }
+        if (EARLY && STAGE == 3) lam_nx = lam_acc(lam_nx, cell_max_eig<DIM>(nu));
+#pragma unroll
+        for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
+        Tc = Tn;
+        __syncthreads();
+        buf ^= 1;
+      }
+      bad_c = tup_negative(Tc);
+    }
+  }
+  for (; m < m1; ++m) {
     const uint32_t idx = col + (uint32_t)m * sm;
-    const double *L = lds + buf * (NTUP * PL);
+    const double *L = lds + buf * (NTUP * PS);
     const uint8_t *LF = lfl + (MASK ? buf * PL : 0);
-    double *Ln = lds + (buf ^ 1) * (NTUP * PL);
+    double *Ln = lds + (buf ^ 1) * (NTUP * PS);
     uint8_t *LFn = lfl + (MASK ? (buf ^ 1) * PL : 0);
     const bool in_chunk = m + 1 < m1;                // plane m + 1 is this chunk's
     const bool has_next = in_chunk || have_beyond;  // a cell above this one
@@ -474,7 +578,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
       bad_n = tup_negative(Tn);
       fn = ff;
       if (in_chunk) {
-        tup_store<PL>(Ln, myslot, Tn);
+        tup_store<PS>(Ln, myslot, Tn);
         if (MASK) LFn[myslot] = fn;
       }
     }
@@ -483,7 +587,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
       Tup Th;
       ph.state(gp, dt, Xh);
       make_tup(Xh, Th);
-      tup_store<PL>(Ln, hslot, Th);
+      tup_store<PS>(Ln, hslot, Th);
       if (MASK) LFn[hslot] = fh;
     }
     // the loads of plane m + 2 (a whole plane of work to arrive)
@@ -515,7 +619,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
             double l;
             if (hm && !wall) {
               Tup Nb;
-              tup_load<PL>(L, sm_, Nb);
+              tup_load<PS>(L, sm_, Nb);
               l = face_flux_area(Nb, Tc, a, pow2A, A, halfA, AFm);
               // counted here only when the neighbour does not evaluate this face
               if (!(fm & CELL_SOLVED) && (bad_c | tup_negative(Nb))) neg.add(Nb, Tc);
@@ -530,7 +634,7 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
             double l;
             if (hp && !wall) {
               Tup Nb;
-              tup_load<PL>(L, sp_, Nb);
+              tup_load<PS>(L, sp_, Nb);
               l = face_flux_area(Tc, Nb, a, pow2A, A, halfA, AFp);
               if (bad_c | tup_negative(Nb)) neg.add(Tc, Nb);
             } else {
@@ -618,8 +722,9 @@ template <int DIM, int TX, int TY, int STAGE, bool MASK, bool P2, bool REFL, boo
 __global__ __launch_bounds__(TX *TY, 2) void k_march(GridDev g, StageIn<STAGE> in, const double *U, double *out,
                                                      Ctrl *ctrl, int mode, int chunk, int ntx, int nty) {
   constexpr int PL = (TY + 2) * (TX + 2);
-  static_assert(2 * NTUP * PL * 8 <= 120 * 1024, "two tuple planes must fit in LDS");
-  __shared__ double lds[2 * NTUP * PL];
+  constexpr int PS = PL + 8;  // march_block's field stride
+  static_assert(2 * NTUP * PS * 8 <= 120 * 1024, "two tuple planes must fit in LDS");
+  __shared__ double lds[2 * NTUP * PS];
   __shared__ uint8_t lfl[MASK ? 2 * PL : 1];
   __shared__ double bct[6 * NTUP];
   constexpr int NT = TX * TY;

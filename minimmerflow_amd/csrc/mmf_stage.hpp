@@ -267,6 +267,11 @@ struct NegCount {
     p += (unsigned)(L.p < 0.) + (unsigned)(R.p < 0.);
     r += (unsigned)(L.c[0] < 0.) + (unsigned)(R.c[0] < 0.);
   }
+  // add(L, R) where c holds, without a branch
+  __device__ __forceinline__ void add_if(bool c, const Tup &L, const Tup &R) {
+    p += c ? (unsigned)(L.p < 0.) + (unsigned)(R.p < 0.) : 0u;
+    r += c ? (unsigned)(L.c[0] < 0.) + (unsigned)(R.c[0] < 0.) : 0u;
+  }
   __device__ __forceinline__ void flush(Ctrl *ctrl, int tid) const {
     unsigned int sp = p, sr = r;
 #pragma unroll
