@@ -59,6 +59,10 @@ CASES = [
      {"boxes": [[0.9, -0.75, -0.9, 1.0, -0.68, -0.8]]}),
     # a body on the periodic z sides: the planes across the wrap need the masked code
     ("3d periodic z body", 3, (70, 18, 40), [0, 3, 1, 0, 4, 4], {"boxes": [[-0.2, -0.9, -1.0, 0.3, -0.7, -0.95]]}),
+    # full 64 x 8 tiles with copy / Dirichlet x and y sides: the flat plane
+    # pass (border faces from LDS slots, counts without branches)
+    ("3d full tiles dirichlet", 3, (128, 16, 20), [3, 0, 0, 3, 0, 3], None),
+    ("3d full tiles periodic z", 3, (64, 24, 30), [0, 3, 3, 0, 4, 4], None),
     ("2d free", 2, (64, 64), [0] * 4, None),
     ("2d ragged periodic", 2, (100, 37), [4, 4, 4, 4], None),
     ("2d box reflecting", 2, (77, 50), [1, 1, 1, 1], {"boxes": [[-0.2, -0.2, -1, 0.3, 0.1, 1]]}),
@@ -112,18 +116,20 @@ def test_march_equals_gather(mmf, orc, name, dim, n, bc, body):
 
 
 # negative densities placed across tile / strip edges: 3D rows 7 | 8 (64 x 8 tiles), 2D columns
-# 61 | 62 (62-cell strips)
-NEG_RHO = {3: [943, 963], 2: [61, 412]}
+# 61 | 62 (62-cell strips); on the full-tile grid (the flat plane pass) also at the x and y
+# borders and across planes
+NEG_RHO = {(3, 20): [943, 963], (2, 70): [61, 412],
+           (3, 64): [2499, 2563, 3392, 4479, 5130, 7114, 7178]}
 
 
-@pytest.mark.parametrize("dim,n", [(3, [20, 20, 20]), (2, [70, 33])])
+@pytest.mark.parametrize("dim,n", [(3, [20, 20, 20]), (2, [70, 33]), (3, [64, 16, 12])])
 def test_march_positivity_counter(mmf, orc, dim, n):
     """Negative pressures and densities are counted per face side as the oracle
     counts them (the marching kernels sum per thread and add once per wave)."""
     s, U = build(mmf, dim, n, [0] * 6)
     U = U.copy()
     U[[5, 777, 2000], 4] = 0.0  # zero energy -> negative temperature and pressure
-    U[NEG_RHO[dim], 0] = -0.5
+    U[NEG_RHO[(dim, n[0])], 0] = -0.5
     h = 2.0 / max(n)
     o = orc.Solver(orc.problem(orc.RADSOD, dim), 1, n, [-1.0] * 3, h, side_bc=[0] * 6)
     o.compute_rhs(U)
