@@ -485,15 +485,6 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
 #pragma unroll
           for (int k = 0; k < NF; ++k) Ucur[k] = Uo.at(k, idx);
         }
-        // halo ring of plane m + 1 (waves 0-2), and its loads of plane m + 2
-        if (hslot >= 0) {
-          double Xh[NF];
-          Tup Th;
-          ph.state(gp, dt, Xh);
-          make_tup(Xh, Th);
-          tup_store<PS>(Ln, hslot, Th);
-          if (m + 2 < m1) ph.issue(src, hcol + (uint32_t)(m + 2) * sm);
-        }
         // plane m + 1's tuple, then the loads of plane m + 2 (plane m + 1 again
         // where no plane m + 2 is needed: an in-bounds load left unused)
         Tup Tn;
@@ -541,6 +532,15 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
 #pragma unroll
         for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
         Tc = Tn;
+        // halo ring of plane m + 1 (waves 0-2), and its loads of plane m + 2
+        if (hslot >= 0) {
+          double Xh[NF];
+          Tup Th;
+          ph.state(gp, dt, Xh);
+          make_tup(Xh, Th);
+          tup_store<PS>(Ln, hslot, Th);
+          if (m + 2 < m1) ph.issue(src, hcol + (uint32_t)(m + 2) * sm);
+        }
         __syncthreads();
         buf ^= 1;
       }
