@@ -532,7 +532,10 @@ __device__ __forceinline__ void march_block(const GridDev &g, StageIn<STAGE> in,
 #pragma unroll
         for (int k = 0; k < NF; ++k) AFlo[k] = AFhi[k];
         Tc = Tn;
-        // halo ring of plane m + 1 (waves 0-2), and its loads of plane m + 2
+        // halo ring of plane m + 1 (waves 0-2) and its loads of plane m + 2,
+        // after the update: those waves start the plane's critical chain (next
+        // tuple, face above, update) right after the barrier, and the halo
+        // tuple is read only after the next one
         if (hslot >= 0) {
           double Xh[NF];
           Tup Th;
