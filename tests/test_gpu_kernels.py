@@ -63,6 +63,7 @@ CASES = [
     # pass (border faces from LDS slots, counts without branches)
     ("3d full tiles dirichlet", 3, (128, 16, 20), [3, 0, 0, 3, 0, 3], None),
     ("3d full tiles periodic z", 3, (64, 24, 30), [0, 3, 3, 0, 4, 4], None),
+    ("3d full tiles all dirichlet", 3, (64, 16, 18), [3] * 6, None),
     ("2d free", 2, (64, 64), [0] * 4, None),
     ("2d ragged periodic", 2, (100, 37), [4, 4, 4, 4], None),
     ("2d box reflecting", 2, (77, 50), [1, 1, 1, 1], {"boxes": [[-0.2, -0.2, -1, 0.3, 0.1, 1]]}),
@@ -73,6 +74,9 @@ CASES = [
 def test_march_equals_gather(mmf, orc, name, dim, n, bc, body):
     info = np.zeros((6, 5))
     info[:] = [1.0, 0.3, -0.1, 0.05 if dim == 3 else 0.0, 1.0]
+    # a different Dirichlet state per side (p, u, v, w, T), so that a side's
+    # state read from another side's slot changes the result
+    info += np.arange(6)[:, None] * np.array([0.05, 0.02, 0.01, -0.01 if dim == 3 else 0.0, 0.03])
     s, U = build(mmf, dim, list(n), bc, info, body)
     ref = run_variant(mmf, s, U, mmf.KERNEL_GATHER)
     shapes = [(64, r, c) for r, c in SHAPES] + SHAPES32 if dim == 3 else [(0, 1, 16), (0, 1, 5), (0, 1, 1000)]
